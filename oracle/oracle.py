@@ -1,0 +1,212 @@
+"""ctypes front end for the oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, always as the checker or the CPU baseline.  It loads
+  * libczw_oracle.so   -- the C restatement of the reference block compressor
+                          (czw_oracle.c; Compressor/source/WaveletsOnInterval.h,
+                          FullWaveletTransform.h, WaveletCompressor.cpp)
+  * _ref/libczref_*.so -- the reference's own compressor compiled from its
+                          sources (only where oracle/Makefile could build it).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BLOCK = 32
+NB3 = BLOCK ** 3
+MASK_BYTES = NB3 // 8
+
+_u8p = C.POINTER(C.c_uint8)
+
+
+def _ptr(a, t=C.c_void_p):
+    return C.cast(a.ctypes.data, t)
+
+
+def _dt(prec):
+    if prec in (4, "f32", np.float32):
+        return np.float32, "f32"
+    if prec in (8, "f64", np.float64):
+        return np.float64, "f64"
+    raise ValueError(f"bad precision {prec!r}")
+
+
+_lib = None
+
+
+def lib():
+    """The C restatement; builds it with oracle/Makefile when it is missing."""
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "libczw_oracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        for s in ("f32", "f64"):
+            getattr(L, f"czo_compress_{s}").restype = C.c_size_t
+            getattr(L, f"czo_compress_{s}").argtypes = [C.c_void_p, C.c_float, C.c_int, C.c_int, C.c_int, C.c_void_p]
+            getattr(L, f"czo_decompress_{s}").restype = C.c_int
+            getattr(L, f"czo_decompress_{s}").argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.c_void_p]
+            getattr(L, f"czo_fwt_{s}").argtypes = [C.c_void_p, C.c_int]
+            getattr(L, f"czo_iwt_{s}").argtypes = [C.c_void_p, C.c_int]
+            getattr(L, f"czo_row_{s}").argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+            getattr(L, f"czo_compress_field_{s}").argtypes = [
+                C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_int,
+                C.c_void_p, C.c_size_t, C.c_void_p]
+            getattr(L, f"czo_decompress_field_{s}").restype = C.c_int
+            getattr(L, f"czo_decompress_field_{s}").argtypes = [
+                C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                C.c_void_p, C.c_size_t, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def build(ref=True):
+    """Compile the oracle (and, where the reference sources exist, oracle/_ref)."""
+    import subprocess
+    target = "all" if ref else "oracle"
+    subprocess.run(["make", "-s", "-C", HERE, target], check=True)
+
+
+def row(x, wtype, forward=True):
+    dt, s = _dt(x.dtype.itemsize)
+    a = np.ascontiguousarray(x, dtype=dt).copy()
+    getattr(lib(), f"czo_row_{s}")(_ptr(a), a.size, wtype, int(bool(forward)))
+    return a
+
+
+def fwt(block, wtype):
+    dt, s = _dt(block.dtype.itemsize)
+    a = np.ascontiguousarray(block, dtype=dt).reshape(NB3).copy()
+    getattr(lib(), f"czo_fwt_{s}")(_ptr(a), wtype)
+    return a.reshape(BLOCK, BLOCK, BLOCK)
+
+
+def iwt(block, wtype):
+    dt, s = _dt(block.dtype.itemsize)
+    a = np.ascontiguousarray(block, dtype=dt).reshape(NB3).copy()
+    getattr(lib(), f"czo_iwt_{s}")(_ptr(a), wtype)
+    return a.reshape(BLOCK, BLOCK, BLOCK)
+
+
+def compress(block, eps, wtype, zerobits=0, shuffle=False):
+    """One block [z][y][x] -> payload bytes (mask 4096 B + survivors)."""
+    dt, s = _dt(block.dtype.itemsize)
+    a = np.ascontiguousarray(block, dtype=dt).reshape(NB3)
+    out = np.empty(MASK_BYTES + NB3 * a.itemsize, np.uint8)
+    n = getattr(lib(), f"czo_compress_{s}")(_ptr(a), float(eps), wtype, zerobits, int(bool(shuffle)), _ptr(out))
+    return out[:n].copy()
+
+
+def decompress(payload, wtype, prec, shuffle=False):
+    dt, s = _dt(prec)
+    p = np.ascontiguousarray(np.frombuffer(bytes(payload), np.uint8))
+    out = np.empty(NB3, dt)
+    rc = getattr(lib(), f"czo_decompress_{s}")(_ptr(p), p.size, wtype, int(bool(shuffle)), _ptr(out))
+    if rc:
+        raise ValueError("payload size disagrees with the mask popcount")
+    return out.reshape(BLOCK, BLOCK, BLOCK)
+
+
+def compress_field(field, eps, wtype, zerobits=0, shuffle=False, b0=0, b1=None):
+    """Blocks [b0,b1) of an h5-order field f[X][Y][Z] -> (slots, nbytes)."""
+    dt, s = _dt(field.dtype.itemsize)
+    f = np.ascontiguousarray(field, dtype=dt)
+    NX, NY, NZ = f.shape
+    nblk = (NX // BLOCK) * (NY // BLOCK) * (NZ // BLOCK)
+    b1 = nblk if b1 is None else b1
+    slot = MASK_BYTES + NB3 * f.itemsize
+    slots = np.empty((b1 - b0) * slot, np.uint8)
+    nbytes = np.empty(b1 - b0, np.uint32)
+    getattr(lib(), f"czo_compress_field_{s}")(_ptr(f), NX, NY, NZ, b0, b1, float(eps), wtype, zerobits,
+                                              int(bool(shuffle)), _ptr(slots), slot, _ptr(nbytes))
+    return slots.reshape(b1 - b0, slot), nbytes
+
+
+def decompress_field(slots, nbytes, shape, wtype, prec, shuffle=False, b0=0, b1=None, out=None):
+    dt, s = _dt(prec)
+    NX, NY, NZ = shape
+    nblk = (NX // BLOCK) * (NY // BLOCK) * (NZ // BLOCK)
+    b1 = nblk if b1 is None else b1
+    f = np.zeros(shape, dt) if out is None else out
+    sl = np.ascontiguousarray(slots)
+    nb = np.ascontiguousarray(nbytes, dtype=np.uint32)
+    rc = getattr(lib(), f"czo_decompress_field_{s}")(_ptr(f), NX, NY, NZ, b0, b1, wtype, int(bool(shuffle)),
+                                                     _ptr(sl), sl.shape[1], _ptr(nb))
+    if rc:
+        raise ValueError("corrupt payload")
+    return f
+
+
+# ---------------------------------------------------------------------------
+# the reference's own compressor (oracle/_ref), when it was built here
+_refs = {}
+
+
+def ref_available(prec="f32", zerobits=0, shuffle=False):
+    return os.path.exists(_ref_path(prec, zerobits, shuffle))
+
+
+def _ref_path(prec, zerobits, shuffle):
+    _, s = _dt(prec)
+    v = s + ("_z4" if zerobits == 4 else "") + ("sh" if shuffle and zerobits == 4 else ("_sh" if shuffle else ""))
+    if zerobits not in (0, 4):
+        raise ValueError("oracle/_ref is built for zerobits 0 and 4 only")
+    return os.path.join(HERE, "_ref", f"libczref_{v}.so")
+
+
+def ref(prec="f32", zerobits=0, shuffle=False):
+    p = _ref_path(prec, zerobits, shuffle)
+    if p not in _refs:
+        L = C.CDLL(p)
+        L.rf_compress.restype = C.c_long
+        L.rf_compress.argtypes = [C.c_void_p, C.c_float, C.c_int, C.c_void_p]
+        L.rf_decompress.restype = C.c_int
+        L.rf_decompress.argtypes = [C.c_void_p, C.c_long, C.c_int, C.c_void_p]
+        L.rf_fwt.argtypes = [C.c_void_p, C.c_int]
+        L.rf_iwt.argtypes = [C.c_void_p, C.c_int]
+        L.rf_row.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.rf_sizeof_compressor.restype = C.c_long
+        _refs[p] = L
+    return _refs[p]
+
+
+def ref_compress(block, eps, wtype, zerobits=0, shuffle=False):
+    dt, s = _dt(block.dtype.itemsize)
+    L = ref(s, zerobits, shuffle)
+    a = np.ascontiguousarray(block, dtype=dt).reshape(NB3)
+    out = np.empty(MASK_BYTES + NB3 * a.itemsize, np.uint8)
+    n = L.rf_compress(_ptr(a), float(eps), wtype, _ptr(out))
+    return out[:n].copy()
+
+
+def ref_decompress(payload, wtype, prec, zerobits=0, shuffle=False):
+    dt, s = _dt(prec)
+    L = ref(s, zerobits, shuffle)
+    p = np.ascontiguousarray(np.frombuffer(bytes(payload), np.uint8))
+    out = np.empty(NB3, dt)
+    L.rf_decompress(_ptr(p), p.size, wtype, _ptr(out))
+    return out.reshape(BLOCK, BLOCK, BLOCK)
+
+
+def ref_fwt(block, wtype):
+    dt, s = _dt(block.dtype.itemsize)
+    a = np.ascontiguousarray(block, dtype=dt).reshape(NB3).copy()
+    ref(s).rf_fwt(_ptr(a), wtype)
+    return a.reshape(BLOCK, BLOCK, BLOCK)
+
+
+def ref_iwt(block, wtype):
+    dt, s = _dt(block.dtype.itemsize)
+    a = np.ascontiguousarray(block, dtype=dt).reshape(NB3).copy()
+    ref(s).rf_iwt(_ptr(a), wtype)
+    return a.reshape(BLOCK, BLOCK, BLOCK)
+
+
+def ref_row(x, wtype, forward=True):
+    dt, s = _dt(x.dtype.itemsize)
+    a = np.ascontiguousarray(x, dtype=dt).copy()
+    ref(s).rf_row(_ptr(a), a.size, wtype, int(bool(forward)))
+    return a

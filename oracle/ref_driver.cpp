@@ -1,0 +1,81 @@
+// ref_driver.cpp -- a small C-ABI shim around the reference's own block
+// compressor, compiled together with the reference sources where they lie
+// (Compressor/source/WaveletCompressor.cpp) by oracle/Makefile into
+// oracle/_ref/.  TEST INFRASTRUCTURE ONLY: used to pin the oracle restatement
+// and to write golden vectors; never part of the product path.
+//
+// The reference selects zerobits/shuffle at compile time (_USE_ZEROBITS_,
+// _ZEROBITS_, _USE_SHUFFLE3_, Tools/Makefile:78-93), so one library is built
+// per variant; every variant exports the same symbols.
+#include <cstring>
+#include <cstdlib>
+#include "WaveletCompressor.h"
+
+extern "C" {
+
+int rf_sizeof_real(void) { return (int)sizeof(Real); }
+long rf_sizeof_compressor(void) { return (long)sizeof(WaveletCompressor); }
+int rf_zerobits(void)
+{
+#if defined(_USE_ZEROBITS_)
+    return _ZEROBITS_;
+#else
+    return 0;
+#endif
+}
+int rf_shuffle(void)
+{
+#if defined(_USE_SHUFFLE3_)
+    return 1;
+#else
+    return 0;
+#endif
+}
+
+// block: [z][y][x] Real; out: >= 4096 + 32768*sizeof(Real) bytes.
+long rf_compress(const Real *block, float threshold, int wtype, unsigned char *out)
+{
+    WaveletCompressor *c = new WaveletCompressor;
+    std::memcpy(&c->uncompressed_data()[0][0][0], block, sizeof(Real) * 32768);
+    const size_t n = c->compress(threshold, false, wtype);
+    std::memcpy(out, c->compressed_data(), n);
+    delete c;
+    return (long)n;
+}
+
+int rf_decompress(const unsigned char *payload, long nbytes, int wtype, Real *block)
+{
+    WaveletCompressor *c = new WaveletCompressor;
+    std::memcpy(c->compressed_data(), payload, (size_t)nbytes);
+    c->decompress(false, (size_t)nbytes, wtype, (Real(*)[32][32])block);
+    delete c;
+    return 0;
+}
+
+void rf_fwt(Real *block, int wtype)
+{
+    WaveletsOnInterval::FullTransform<32> *f = new WaveletsOnInterval::FullTransform<32>;
+    std::memcpy(&f->data[0][0][0], block, sizeof(Real) * 32768);
+    f->fwt(wtype);
+    std::memcpy(block, &f->data[0][0][0], sizeof(Real) * 32768);
+    delete f;
+}
+
+void rf_iwt(Real *block, int wtype)
+{
+    WaveletsOnInterval::FullTransform<32> *f = new WaveletsOnInterval::FullTransform<32>;
+    std::memcpy(&f->data[0][0][0], block, sizeof(Real) * 32768);
+    f->iwt(wtype);
+    std::memcpy(block, &f->data[0][0][0], sizeof(Real) * 32768);
+    delete f;
+}
+
+void rf_row(Real *row, int n, int wtype, int forward)
+{
+    using WaveletsOnInterval::WI4;
+    if (n == 8) { if (forward) WI4::transform<8, true>(row, wtype); else WI4::transform<8, false>(row, wtype); }
+    if (n == 16) { if (forward) WI4::transform<16, true>(row, wtype); else WI4::transform<16, false>(row, wtype); }
+    if (n == 32) { if (forward) WI4::transform<32, true>(row, wtype); else WI4::transform<32, false>(row, wtype); }
+}
+
+} // extern "C"
