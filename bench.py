@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark: CubismZ wavelet compression stage on one MI355X.
+
+Workload (the metric's configuration): one 512^3 float32 cavitation-like
+pressure field (background 1.0 + 200 bubbles, cubismz_amd.fields.bubble_field,
+seed 1), 16^3 blocks of 32^3, eps = 1e-3, wavelet type 3 (the reference CLI
+default, Tools/Test_IO_Compressed.h:305), no zerobits, no shuffle.
+
+A step = compress (fwt + threshold + mask + survivor stream + record framing)
++ decompress (decode + iwt) of the whole field, input already resident in
+HBM.  value = field bytes per step / step time, in GB/s.
+
+Also reported:
+  roofline      dominant kernel's algorithmic HBM bytes per launch / its
+                average duration (HIP events on the launch stream), vs the
+                8 TB/s HBM peak.
+  cpu_baseline  the oracle (C restatement of the reference, OpenMP) on the
+                same field on this host's cores.
+  parity        GPU records and decoded field compared bit for bit with the
+                oracle for every block.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--eps", type=float, default=1e-3)
+    ap.add_argument("--wtype", type=int, default=3)
+    ap.add_argument("--zerobits", type=int, default=0)
+    ap.add_argument("--shuffle", type=int, default=0)
+    ap.add_argument("--bubbles", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full parity check")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "blocks are independent, so N GPUs would each compress their own subdomain.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from cubismz_amd import czw, fields
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    n = a.size
+    shape = (n, n, n)
+    prec = 4
+    nblocks = (n // 32) ** 3
+    print(f"[bench] generating {shape} bubble field", file=sys.stderr, flush=True)
+    field = fields.bubble_field(shape, seed=1, nbubbles=a.bubbles, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    slot = czw.slot_bytes(prec)
+    slots = torch.empty(nblocks * slot, dtype=torch.uint8, device=dev)
+    nbytes = torch.empty(nblocks, dtype=torch.int32, device=dev)
+    records = torch.empty(czw.records_bound(nblocks, prec), dtype=torch.uint8, device=dev)
+    offsets = torch.empty(nblocks + 1, dtype=torch.int64, device=dev)
+    out = torch.empty_like(field)
+    err = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    def step(ev=None):
+        if ev:
+            ev[0].record()
+        czw.compress_field_slots(field, a.eps, a.wtype, a.zerobits, a.shuffle, slots, nbytes)
+        if ev:
+            ev[1].record()
+        czw.pack_records(slots, slot, nbytes, nblocks, records, offsets)
+        if ev:
+            ev[2].record()
+        czw.decompress_field(records, offsets, shape, torch.float32, a.wtype, a.shuffle, out=out, err=err,
+                             check=False)
+        if ev:
+            ev[3].record()
+
+    print(f"[bench] warmup {a.warmup}", file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    if int(err.item()):
+        raise SystemExit("decoder reported corrupt records")
+
+    # timed region: K steps, per-stage HIP events on the launch stream
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(a.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_c = sum(e[0].elapsed_time(e[1]) for e in evs) / a.steps * 1e-3
+    t_p = sum(e[1].elapsed_time(e[2]) for e in evs) / a.steps * 1e-3
+    t_d = sum(e[2].elapsed_time(e[3]) for e in evs) / a.steps * 1e-3
+    t_step = sum(e[0].elapsed_time(e[3]) for e in evs) / a.steps * 1e-3
+    ms_per_step = wall / a.steps * 1e3
+
+    field_bytes = field.numel() * prec
+    off = offsets.cpu().numpy()
+    nb = nbytes.cpu().numpy().astype(np.int64)
+    payload = int(nb.sum())
+    surv = int(((nb - 4096) // prec).sum())
+    rho = surv / field.numel()
+    # algorithmic bytes (per launch): compress reads the field and writes
+    # 4 + 4096 + s*S_b per block; decompress reads that and writes the field
+    alg_c = field_bytes + payload
+    alg_d = field_bytes + payload + 4 * nblocks
+    kern = {"k_compress_f32": (alg_c, t_c), "k_decompress_f32": (alg_d, t_d)}
+    dom = max(kern, key=lambda k: kern[k][1])
+    achieved = kern[dom][0] / kern[dom][1]
+
+    recs = records[: int(off[-1])].cpu().numpy()
+    outh = out.cpu().numpy()
+    fh = field.cpu().numpy()
+    err_inf = float(np.max(np.abs(outh.astype(np.float64) - fh)))
+    mse = float(np.mean((outh.astype(np.float64) - fh) ** 2))
+    psnr = 20 * np.log10((fh.max() - fh.min()) / (2 * np.sqrt(mse))) if mse > 0 else float("inf")
+
+    cpu = None
+    parity = None
+    if not a.no_cpu:
+        from oracle import oracle as O
+
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = O.set_threads(threads)
+        print(f"[bench] cpu baseline (oracle, {threads} threads) + parity over {nblocks} blocks", file=sys.stderr,
+              flush=True)
+        tc0 = time.perf_counter()
+        oslots, onb = O.compress_field(fh, a.eps, a.wtype, a.zerobits, a.shuffle)
+        tc1 = time.perf_counter()
+        ofield = O.decompress_field(oslots, onb, shape, a.wtype, "f32", a.shuffle)
+        tc2 = time.perf_counter()
+        cpu = {"value": round(field_bytes / (tc2 - tc0) / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+               "sample": f"whole {n}^3 field ({nblocks} blocks), compress {tc1 - tc0:.3f} s + decompress "
+                         f"{tc2 - tc1:.3f} s"}
+        mism = 0
+        for b in range(nblocks):
+            o = int(off[b])
+            if int(recs[o:o + 4].view(np.int32)[0]) != int(onb[b]) or \
+                    not np.array_equal(recs[o + 4:o + 4 + int(onb[b])], oslots[b, :onb[b]]):
+                mism += 1
+        parity = {"blocks": nblocks, "records_bitexact": mism == 0, "blocks_differing": mism,
+                  "decoded_bitexact": bool(np.array_equal(outh.view(np.uint32), ofield.view(np.uint32)))}
+
+    res = {
+        "metric": "GB/s compress+decompress 512^3 fp32 eps=1e-3 (bit-exact); batched & adjoint GB/s",
+        "value": round(field_bytes / t_step / 1e9, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(t_step * 1e3, 4),
+        "wall_ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain compress+decompress, {n}^3 f32 bubble field ({a.bubbles} bubbles), "
+                               f"{nblocks} blocks of 32^3",
+                   "eps": a.eps, "wtype": a.wtype, "zerobits": a.zerobits, "shuffle": a.shuffle},
+        "stages_ms": {"compress": round(t_c * 1e3, 4), "pack": round(t_p * 1e3, 4), "decompress": round(t_d * 1e3, 4)},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "alg_bytes_per_launch": kern[dom][0], "avg_launch_ms": round(kern[dom][1] * 1e3, 4)},
+        "step_roofline_frac": round((alg_c + alg_d) / t_step / HBM_PEAK, 4),
+        "compression": {"payload_bytes": payload, "survivor_fraction": round(rho, 5),
+                        "ratio_vs_records": round(field_bytes / float(off[-1]), 3),
+                        "linf": err_inf, "psnr_db": round(float(psnr), 4)},
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

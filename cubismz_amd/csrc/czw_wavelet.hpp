@@ -1,0 +1,197 @@
+// czw_wavelet.hpp -- register-resident one-level 1D wavelet transforms on rows
+// of 32, 16 or 8 values, for the gfx950 kernels.
+//
+// Arithmetic follows the reference stencils exactly
+// (Compressor/source/WaveletsOnInterval.h:33-201):
+//  * every predictor is a sum of products of Real values by binary64
+//    constants, evaluated left to right in binary64 and rounded to Real on
+//    return;
+//  * differences, sums and the 0.5 scalings around the predictors are Real
+//    operations.
+// For Real = float every product of a float by these constants (at most 6
+// significant bits) is exact in binary64, so fma(k, x, t) rounds exactly the
+// same value as the reference's t + k*x; the float path uses fma to save one
+// instruction per term.  For Real = double the products are inexact and the
+// double path keeps separate multiplies and adds (the library is compiled
+// with -ffp-contract=off).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace czw {
+
+#define CZW_DI __device__ __forceinline__
+
+template <typename R> struct Pred;
+
+template <> struct Pred<float> {
+    typedef float R;
+    static CZW_DI float rnd(double t) { return (float)t; }
+    // wtype 1/2: WaveletsOnInterval.h:33-51
+    static CZW_DI float p4_first(R a, R b, R c, R e)
+    {
+        double t = 0.3125 * (double)a;
+        t = __builtin_fma(0.9375, (double)b, t);
+        t = __builtin_fma(-0.3125, (double)c, t);
+        t = __builtin_fma(0.0625, (double)e, t);
+        return rnd(t);
+    }
+    static CZW_DI float p4_mid(R a, R b, R c, R e)
+    {
+        double t = -0.0625 * (double)a;
+        t = __builtin_fma(0.5625, (double)b, t);
+        t = __builtin_fma(0.5625, (double)c, t);
+        t = __builtin_fma(-0.0625, (double)e, t);
+        return rnd(t);
+    }
+    static CZW_DI float p4_otl(R a, R b, R c, R e)
+    {
+        double t = 0.0625 * (double)a;
+        t = __builtin_fma(-0.3125, (double)b, t);
+        t = __builtin_fma(0.9375, (double)c, t);
+        t = __builtin_fma(0.3125, (double)e, t);
+        return rnd(t);
+    }
+    static CZW_DI float p4_last(R a, R b, R c, R e)
+    {
+        double t = -0.3125 * (double)a;
+        t = __builtin_fma(1.3125, (double)b, t);
+        t = __builtin_fma(-2.1875, (double)c, t);
+        t = __builtin_fma(2.1875, (double)e, t);
+        return rnd(t);
+    }
+    // wtype 3: WaveletsOnInterval.h:55-83
+    static CZW_DI float q0_first(R a, R b, R c)
+    {
+        double t = 1.375 * (double)a;
+        t = __builtin_fma(-0.5, (double)b, t);
+        return rnd(__builtin_fma(0.125, (double)c, t));
+    }
+    static CZW_DI float q1_first(R a, R b, R c)
+    {
+        double t = 0.625 * (double)a;
+        t = __builtin_fma(0.5, (double)b, t);
+        return rnd(__builtin_fma(-0.125, (double)c, t));
+    }
+    static CZW_DI float q0_mid(R a, R b, R c)
+    {
+        const double t = __builtin_fma(0.125, (double)a, (double)b);
+        return rnd(__builtin_fma(-0.125, (double)c, t));
+    }
+    static CZW_DI float q1_mid(R a, R b, R c)
+    {
+        const double t = __builtin_fma(-0.125, (double)a, (double)b);
+        return rnd(__builtin_fma(0.125, (double)c, t));
+    }
+    static CZW_DI float q0_last(R a, R b, R c)
+    {
+        double t = -0.125 * (double)a;
+        t = __builtin_fma(0.5, (double)b, t);
+        return rnd(__builtin_fma(0.625, (double)c, t));
+    }
+    static CZW_DI float q1_last(R a, R b, R c)
+    {
+        double t = 0.125 * (double)a;
+        t = __builtin_fma(-0.5, (double)b, t);
+        return rnd(__builtin_fma(1.375, (double)c, t));
+    }
+    // wtype 2 lifting update s +- 0.5*d in binary64 (WaveletsOnInterval.h:124-126,137-139)
+    static CZW_DI float lift_add(R s, R d) { return rnd(__builtin_fma(0.5, (double)d, (double)s)); }
+    static CZW_DI float lift_sub(R s, R d) { return rnd(__builtin_fma(-0.5, (double)d, (double)s)); }
+};
+
+template <> struct Pred<double> {
+    typedef double R;
+    static CZW_DI double p4_first(R a, R b, R c, R e) { return ((0.3125 * a + 0.9375 * b) - 0.3125 * c) + 0.0625 * e; }
+    static CZW_DI double p4_mid(R a, R b, R c, R e) { return ((-0.0625 * a + 0.5625 * b) + 0.5625 * c) - 0.0625 * e; }
+    static CZW_DI double p4_otl(R a, R b, R c, R e) { return ((0.0625 * a - 0.3125 * b) + 0.9375 * c) + 0.3125 * e; }
+    static CZW_DI double p4_last(R a, R b, R c, R e) { return ((-0.3125 * a + 1.3125 * b) - 2.1875 * c) + 2.1875 * e; }
+    static CZW_DI double q0_first(R a, R b, R c) { return (1.375 * a - 0.5 * b) + 0.125 * c; }
+    static CZW_DI double q1_first(R a, R b, R c) { return (0.625 * a + 0.5 * b) - 0.125 * c; }
+    static CZW_DI double q0_mid(R a, R b, R c) { return (0.125 * a + b) - 0.125 * c; }
+    static CZW_DI double q1_mid(R a, R b, R c) { return (-0.125 * a + b) + 0.125 * c; }
+    static CZW_DI double q0_last(R a, R b, R c) { return (-0.125 * a + 0.5 * b) + 0.625 * c; }
+    static CZW_DI double q1_last(R a, R b, R c) { return (0.125 * a - 0.5 * b) + 1.375 * c; }
+    static CZW_DI double lift_add(R s, R d) { return s + 0.5 * d; }
+    static CZW_DI double lift_sub(R s, R d) { return s - 0.5 * d; }
+};
+
+// One forward level on d[0..N): result [scalings | details]
+// (WaveletsOnInterval.h:99-130 for wtype 1/2, :156-180 for wtype 0/3).
+template <int N, int WT, typename R>
+CZW_DI void fwd_row(R *d)
+{
+    typedef Pred<R> P;
+    constexpr int H = N / 2;
+    R s[H], det[H];
+    if constexpr (WT == 1 || WT == 2) {
+        det[0] = d[1] - P::p4_first(d[0], d[2], d[4], d[6]);
+#pragma unroll
+        for (int i = 1; i < H - 2; ++i)
+            det[i] = d[2 * i + 1] - P::p4_mid(d[2 * i - 2], d[2 * i], d[2 * i + 2], d[2 * i + 4]);
+        det[H - 2] = d[N - 3] - P::p4_otl(d[N - 8], d[N - 6], d[N - 4], d[N - 2]);
+        det[H - 1] = d[N - 1] - P::p4_last(d[N - 8], d[N - 6], d[N - 4], d[N - 2]);
+#pragma unroll
+        for (int i = 0; i < H; ++i) s[i] = (WT == 2) ? P::lift_add(d[2 * i], det[i]) : d[2 * i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < H; ++i) s[i] = (R)0.5 * (d[2 * i] + d[2 * i + 1]);
+        det[0] = (R)0.5 * ((d[1] - d[0]) - (P::q1_first(s[0], s[1], s[2]) - P::q0_first(s[0], s[1], s[2])));
+#pragma unroll
+        for (int i = 1; i < H - 1; ++i)
+            det[i] = (R)0.5 * ((d[2 * i + 1] - d[2 * i]) -
+                               (P::q1_mid(s[i - 1], s[i], s[i + 1]) - P::q0_mid(s[i - 1], s[i], s[i + 1])));
+        det[H - 1] = (R)0.5 * ((d[N - 1] - d[N - 2]) -
+                               (P::q1_last(s[H - 3], s[H - 2], s[H - 1]) - P::q0_last(s[H - 3], s[H - 2], s[H - 1])));
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        d[i] = s[i];
+        d[H + i] = det[i];
+    }
+}
+
+// One inverse level (WaveletsOnInterval.h:131-152 for wtype 1/2, :181-198 for 0/3).
+template <int N, int WT, typename R>
+CZW_DI void inv_row(R *d)
+{
+    typedef Pred<R> P;
+    constexpr int H = N / 2;
+    R s[H], det[H];
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        s[i] = d[i];
+        det[i] = d[H + i];
+    }
+    if constexpr (WT == 1 || WT == 2) {
+        if constexpr (WT == 2) {
+#pragma unroll
+            for (int i = 0; i < H; ++i) s[i] = P::lift_sub(s[i], det[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < H; ++i) d[2 * i] = s[i];
+        d[1] = P::p4_first(s[0], s[1], s[2], s[3]) + det[0];
+#pragma unroll
+        for (int i = 1; i < H - 2; ++i) d[2 * i + 1] = P::p4_mid(s[i - 1], s[i], s[i + 1], s[i + 2]) + det[i];
+        d[N - 3] = P::p4_otl(s[H - 4], s[H - 3], s[H - 2], s[H - 1]) + det[H - 2];
+        d[N - 1] = P::p4_last(s[H - 4], s[H - 3], s[H - 2], s[H - 1]) + det[H - 1];
+    } else {
+        d[0] = P::q0_first(s[0], s[1], s[2]) - det[0];
+        d[1] = P::q1_first(s[0], s[1], s[2]) + det[0];
+#pragma unroll
+        for (int i = 1; i < H - 1; ++i) {
+            d[2 * i] = P::q0_mid(s[i - 1], s[i], s[i + 1]) - det[i];
+            d[2 * i + 1] = P::q1_mid(s[i - 1], s[i], s[i + 1]) + det[i];
+        }
+        d[N - 2] = P::q0_last(s[H - 3], s[H - 2], s[H - 1]) - det[H - 1];
+        d[N - 1] = P::q1_last(s[H - 3], s[H - 2], s[H - 1]) + det[H - 1];
+    }
+}
+
+template <int N, bool FWD, int WT, typename R>
+CZW_DI void row(R *d)
+{
+    if constexpr (FWD) fwd_row<N, WT, R>(d);
+    else inv_row<N, WT, R>(d);
+}
+
+} // namespace czw

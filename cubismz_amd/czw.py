@@ -1,0 +1,235 @@
+"""Python front end of the gfx950 wavelet block compressor (lib/libczw.so).
+
+Mirrors the per-block compressor of CubismZ,
+``WaveletCompressorGeneric<32,Real>::compress / decompress``
+(Compressor/source/WaveletCompressor.cpp:153-259), batched over every block
+of a field.  Device memory and streams come from torch; all arithmetic runs
+in the HIP kernels behind the C ABI of ``include/czw.h``.  There is no CPU
+fallback: every entry point raises :class:`CZWError` when the library or a
+GPU is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "lib", "libczw.so")
+BLOCK = 32
+NB3 = BLOCK ** 3
+MASK_BYTES = NB3 // 8
+
+# FullWaveletTransform.h:28-35
+WAVELET_NAMES = {
+    0: "None",
+    1: "InterpWavelet4thOrder",
+    2: "LiftedInterpWavelet4thOrder",
+    3: "AverageInterpWavelet3rdOrder",
+}
+
+_STATUS = {0: "ok", -1: "invalid argument", -2: "HIP launch failure", -3: "unsupported combination"}
+
+
+class CZWError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library():
+    """Load lib/libczw.so (built by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise CZWError(f"{LIBPATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIBPATH)
+    vp, i, f, sz = C.c_void_p, C.c_int, C.c_float, C.c_size_t
+    L.czw_version.restype = i
+    L.czw_compress_workspace_bytes.restype = sz
+    L.czw_compress_workspace_bytes.argtypes = [i, i]
+    L.czw_records_bound.restype = sz
+    L.czw_records_bound.argtypes = [i, i]
+    L.czw_compress_field.restype = i
+    L.czw_compress_field.argtypes = [vp, i, i, i, i, f, i, i, i, vp, vp, vp, vp]
+    L.czw_slot_bytes.restype = sz
+    L.czw_slot_bytes.argtypes = [i]
+    L.czw_compress_field_slots.restype = i
+    L.czw_compress_field_slots.argtypes = [vp, i, i, i, i, f, i, i, i, vp, vp, vp]
+    L.czw_pack_records.restype = i
+    L.czw_pack_records.argtypes = [vp, sz, vp, i, vp, vp, vp]
+    L.czw_compress_blocks.restype = i
+    L.czw_compress_blocks.argtypes = [vp, i, i, f, i, i, i, vp, vp, vp, vp]
+    L.czw_decompress_field.restype = i
+    L.czw_decompress_field.argtypes = [vp, vp, i, i, i, i, i, i, vp, vp, vp]
+    L.czw_decompress_blocks.restype = i
+    L.czw_decompress_blocks.argtypes = [vp, vp, i, i, i, i, vp, vp, vp]
+    _lib = L
+    return L
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise CZWError("no GPU visible: the wavelet compressor runs only on the HIP device")
+    return torch
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise CZWError(f"{what}: {_STATUS.get(rc, rc)}")
+
+
+def _stream(torch):
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _prec(dtype, torch):
+    if dtype == torch.float32:
+        return 4
+    if dtype == torch.float64:
+        return 8
+    raise CZWError(f"unsupported dtype {dtype}")
+
+
+def workspace_bytes(nblocks, prec):
+    return int(load_library().czw_compress_workspace_bytes(int(nblocks), int(prec)))
+
+
+def records_bound(nblocks, prec):
+    return int(load_library().czw_records_bound(int(nblocks), int(prec)))
+
+
+def _buffers(torch, device, nblocks, prec, records, offsets, workspace):
+    if records is None:
+        records = torch.empty(records_bound(nblocks, prec), dtype=torch.uint8, device=device)
+    if offsets is None:
+        offsets = torch.empty(nblocks + 1, dtype=torch.int64, device=device)
+    if workspace is None:
+        workspace = torch.empty(workspace_bytes(nblocks, prec), dtype=torch.uint8, device=device)
+    if records.numel() < records_bound(nblocks, prec) or offsets.numel() < nblocks + 1:
+        raise CZWError("records/offsets buffers too small")
+    if workspace.numel() < workspace_bytes(nblocks, prec):
+        raise CZWError("workspace too small")
+    return records, offsets, workspace
+
+
+def compress_field(field, eps, wtype=3, zerobits=0, shuffle=False, records=None, offsets=None, workspace=None):
+    """Compress an HDF5-ordered device field f[NX][NY][NZ] (every edge a multiple of 32).
+
+    Returns (records, offsets): the record stream ``[int32 nbytes][payload]``
+    per block (device uint8) and the nblocks+1 record offsets (device int64).
+    """
+    torch = _torch()
+    L = load_library()
+    if field.dim() != 3 or not field.is_cuda or not field.is_contiguous():
+        raise CZWError("field must be a contiguous 3D device tensor")
+    nx, ny, nz = (int(s) for s in field.shape)
+    prec = _prec(field.dtype, torch)
+    nblocks = (nx // BLOCK) * (ny // BLOCK) * (nz // BLOCK)
+    records, offsets, workspace = _buffers(torch, field.device, nblocks, prec, records, offsets, workspace)
+    rc = L.czw_compress_field(C.c_void_p(field.data_ptr()), nx, ny, nz, prec, float(eps), int(wtype), int(zerobits),
+                              int(bool(shuffle)), C.c_void_p(records.data_ptr()), C.c_void_p(offsets.data_ptr()),
+                              C.c_void_p(workspace.data_ptr()), _stream(torch))
+    _check(rc, "czw_compress_field")
+    return records, offsets
+
+
+def compress_field_slots(field, eps, wtype, zerobits, shuffle, slots, nbytes):
+    """Stage 1 of compress_field: per-block payloads into fixed-size slots."""
+    torch = _torch()
+    L = load_library()
+    nx, ny, nz = (int(s) for s in field.shape)
+    rc = L.czw_compress_field_slots(C.c_void_p(field.data_ptr()), nx, ny, nz, _prec(field.dtype, torch), float(eps),
+                                    int(wtype), int(zerobits), int(bool(shuffle)), C.c_void_p(slots.data_ptr()),
+                                    C.c_void_p(nbytes.data_ptr()), _stream(torch))
+    _check(rc, "czw_compress_field_slots")
+
+
+def pack_records(slots, slot_bytes, nbytes, nblocks, records, offsets):
+    """Stage 2 of compress_field: frame slot payloads into the record stream."""
+    torch = _torch()
+    rc = load_library().czw_pack_records(C.c_void_p(slots.data_ptr()), int(slot_bytes), C.c_void_p(nbytes.data_ptr()),
+                                         int(nblocks), C.c_void_p(records.data_ptr()), C.c_void_p(offsets.data_ptr()),
+                                         _stream(torch))
+    _check(rc, "czw_pack_records")
+
+
+def slot_bytes(prec):
+    return int(load_library().czw_slot_bytes(int(prec)))
+
+
+def compress_blocks(blocks, eps, wtype=3, zerobits=0, shuffle=False, records=None, offsets=None, workspace=None):
+    """Compress device blocks [nb][32][32][32] ([z][y][x]); same outputs as compress_field."""
+    torch = _torch()
+    L = load_library()
+    if blocks.dim() != 4 or tuple(blocks.shape[1:]) != (BLOCK,) * 3 or not blocks.is_cuda or not blocks.is_contiguous():
+        raise CZWError("blocks must be a contiguous device tensor [nb][32][32][32]")
+    nblocks = int(blocks.shape[0])
+    prec = _prec(blocks.dtype, torch)
+    records, offsets, workspace = _buffers(torch, blocks.device, nblocks, prec, records, offsets, workspace)
+    rc = L.czw_compress_blocks(C.c_void_p(blocks.data_ptr()), nblocks, prec, float(eps), int(wtype), int(zerobits),
+                               int(bool(shuffle)), C.c_void_p(records.data_ptr()), C.c_void_p(offsets.data_ptr()),
+                               C.c_void_p(workspace.data_ptr()), _stream(torch))
+    _check(rc, "czw_compress_blocks")
+    return records, offsets
+
+
+def _err_buffer(torch, device, err):
+    if err is None:
+        err = torch.zeros(1, dtype=torch.int32, device=device)
+    return err
+
+
+def _raise_err(err):
+    code = int(err.item())
+    if code:
+        raise CZWError(f"corrupt record stream (device error bits {code:#x})")
+
+
+def decompress_field(records, offsets, shape, dtype, wtype=3, shuffle=False, out=None, err=None, check=True):
+    """Decode every record into an HDF5-ordered field of the given shape."""
+    torch = _torch()
+    L = load_library()
+    nx, ny, nz = (int(s) for s in shape)
+    prec = _prec(dtype, torch)
+    if out is None:
+        out = torch.empty((nx, ny, nz), dtype=dtype, device=records.device)
+    err = _err_buffer(torch, records.device, err)
+    rc = L.czw_decompress_field(C.c_void_p(records.data_ptr()), C.c_void_p(offsets.data_ptr()), nx, ny, nz, prec,
+                                int(wtype), int(bool(shuffle)), C.c_void_p(out.data_ptr()),
+                                C.c_void_p(err.data_ptr()), _stream(torch))
+    _check(rc, "czw_decompress_field")
+    if check:
+        _raise_err(err)
+    return out
+
+
+def decompress_blocks(records, offsets, nblocks, dtype, wtype=3, shuffle=False, out=None, err=None, check=True):
+    """Decode records into device blocks [nb][32][32][32]."""
+    torch = _torch()
+    L = load_library()
+    prec = _prec(dtype, torch)
+    if out is None:
+        out = torch.empty((nblocks, BLOCK, BLOCK, BLOCK), dtype=dtype, device=records.device)
+    err = _err_buffer(torch, records.device, err)
+    rc = L.czw_decompress_blocks(C.c_void_p(records.data_ptr()), C.c_void_p(offsets.data_ptr()), int(nblocks), prec,
+                                 int(wtype), int(bool(shuffle)), C.c_void_p(out.data_ptr()),
+                                 C.c_void_p(err.data_ptr()), _stream(torch))
+    _check(rc, "czw_decompress_blocks")
+    if check:
+        _raise_err(err)
+    return out
+
+
+def records_to_payloads(records, offsets):
+    """Split a host record stream into per-block payload byte strings."""
+    rec = np.asarray(records, dtype=np.uint8)
+    off = np.asarray(offsets, dtype=np.int64)
+    out = []
+    for b in range(len(off) - 1):
+        n = int(rec[off[b]:off[b] + 4].view(np.int32)[0])
+        out.append(rec[off[b] + 4:off[b] + 4 + n].tobytes())
+    return out

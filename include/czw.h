@@ -1,0 +1,120 @@
+/*
+ * czw.h -- C ABI of the MI355X (gfx950) wavelet block compressor.
+ *
+ * This is the device-side replacement for CubismZ's per-block compressor
+ * WaveletCompressorGeneric<32,Real> (Compressor/source/WaveletCompressor.h:35-94)
+ * and for the per-block loops that drive it: the writer loop
+ * SerializerIO_WaveletCompression_MPI_SimpleBlocking::_compress
+ * (Compressor/source/SerializerIO_WaveletCompression_MPI_Simple.h:187-345) and
+ * the reader Reader_WaveletCompression::load_block2
+ * (Compressor/source/Reader_WaveletCompression.h:635-754).  Instead of one
+ * 32^3 block per call, every entry point handles a batch of blocks on one HIP
+ * stream.
+ *
+ * Conventions
+ *  - All data pointers are DEVICE pointers owned by the caller; the library
+ *    allocates nothing in these calls.
+ *  - prec is sizeof(Real): 4 (the reference's float build) or 8 (double build).
+ *  - wtype follows the reference: 1 = InterpWavelet4thOrder,
+ *    2 = LiftedInterpWavelet4thOrder, 3 = AverageInterpWavelet3rdOrder,
+ *    0 = "None" (computes like 3, FullWaveletTransform.h:28-35).
+ *  - eps is a float as in WaveletCompressorGeneric::compress(const float ...)
+ *    (WaveletCompressor.h:61); the double build compares against (double)eps.
+ *  - zerobits in {0,4,8,12,16} clears the low bits of the first 32-bit word
+ *    of every survivor (WaveletCompressor.cpp:34-43,166-173); shuffle != 0
+ *    applies the byte-plane shuffle (WaveletCompressor.cpp:56-93).
+ *  - Calls are asynchronous on `stream`; they return a status code and never
+ *    abort.  Concurrent calls on different streams are safe if they do not
+ *    share buffers.
+ *
+ * Record stream ("records")
+ *   For block b, at byte offsets[b]:  int32 nbytes, then the payload of
+ *   nbytes bytes = [mask 4096 B, bit p at byte p>>3 bit p&7][S survivors].
+ *   offsets has nblocks+1 entries (offsets[nblocks] = total bytes).  This is
+ *   byte-for-byte the per-thread chunk buffer of the reference writer
+ *   (SerializerIO_WaveletCompression_MPI_Simple.h:237-240), so any run of
+ *   consecutive records can be handed to zlib as one .cz chunk.
+ *
+ * Field layout ("field" entry points)
+ *   An HDF5-ordered field f[NX][NY][NZ] (NZ fastest), each dimension a
+ *   multiple of 32.  Block (ix,iy,iz) holds block[z][y][x] =
+ *   f[32ix+x][32iy+y][32iz+z] (Tools/Test_IO_Compressed.h:200-212) and blocks
+ *   are numbered b = ix + bx*(iy + by*iz) (Cubism Grid.h:68).
+ * Block layout ("blocks" entry points)
+ *   blocks[nblocks][32][32][32] in the reference's uncompressed_data() order
+ *   [z][y][x], x fastest (WaveletCompressor.h:57).
+ */
+#ifndef CZW_H
+#define CZW_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CZW_OK = 0,
+    CZW_EINVAL = -1,      /* bad argument (sizes, wtype, zerobits, null) */
+    CZW_EHIP = -2,        /* a HIP launch failed */
+    CZW_EUNSUPPORTED = -3 /* combination not built */
+};
+
+/* Device error bits written by the decompressors into *err (OR-ed). */
+enum {
+    CZW_ERR_NBYTES = 1,   /* record size not 4096 + k*prec within bounds */
+    CZW_ERR_POPCOUNT = 2  /* mask popcount differs from the survivor count */
+};
+
+/* Library version (major*100 + minor). */
+int czw_version(void);
+
+/* Bytes of scratch the compressors need for nblocks blocks. */
+size_t czw_compress_workspace_bytes(int nblocks, int prec);
+
+/* Upper bound of the record stream size for nblocks blocks. */
+size_t czw_records_bound(int nblocks, int prec);
+
+/* Replaces the loop body of SerializerIO_..._MPI_Simple.h:212-240 (gather a
+ * block, WaveletCompressorGeneric::compress at WaveletCompressor.cpp:153-182,
+ * frame the record) for every block of an HDF5-ordered field. */
+int czw_compress_field(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype,
+                       int zerobits, int shuffle, void *records, uint64_t *offsets, void *workspace,
+                       hipStream_t stream);
+
+/* The two stages of czw_compress_field, for callers that want fixed-size
+ * per-block slots or time the stages apart.  Stage 1 writes block b's payload
+ * to slots + b*czw_slot_bytes(prec) and its size to nbytes[b]; stage 2 frames
+ * the payloads into the contiguous record stream (exclusive scan of
+ * 4 + nbytes[b] into offsets, then a copy). */
+size_t czw_slot_bytes(int prec);
+int czw_compress_field_slots(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype,
+                             int zerobits, int shuffle, void *slots, uint32_t *nbytes, hipStream_t stream);
+int czw_pack_records(const void *slots, size_t slot_bytes, const uint32_t *nbytes, int nblocks, void *records,
+                     uint64_t *offsets, hipStream_t stream);
+
+/* Same as czw_compress_field for blocks already laid out [nb][z][y][x]. */
+int czw_compress_blocks(const void *blocks, int nblocks, int prec, float eps, int wtype, int zerobits,
+                        int shuffle, void *records, uint64_t *offsets, void *workspace,
+                        hipStream_t stream);
+
+/* Replaces WaveletCompressorGeneric::decompress (WaveletCompressor.cpp:232-259)
+ * plus copy_to (WaveletCompressor.h:73-82) for every record, writing an
+ * HDF5-ordered field.  offsets[b] points at record b's int32 size word; the
+ * records need not be contiguous.  *err (device) is OR-ed with CZW_ERR_* for
+ * corrupt records, whose blocks are left unwritten. */
+int czw_decompress_field(const void *records, const uint64_t *offsets, int nx, int ny, int nz,
+                         int prec, int wtype, int shuffle, void *field, uint32_t *err,
+                         hipStream_t stream);
+
+/* Same as czw_decompress_field writing blocks [nb][z][y][x]. */
+int czw_decompress_blocks(const void *records, const uint64_t *offsets, int nblocks, int prec,
+                          int wtype, int shuffle, void *blocks, uint32_t *err, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CZW_H */

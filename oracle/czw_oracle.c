@@ -31,3 +31,12 @@
 #undef REAL
 
 int czo_version(void) { return 1; }
+
+#ifdef _OPENMP
+#include <omp.h>
+void czo_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int czo_get_threads(void) { return omp_get_max_threads(); }
+#else
+void czo_set_threads(int n) { (void)n; }
+int czo_get_threads(void) { return 1; }
+#endif

@@ -59,8 +59,16 @@ def lib():
             getattr(L, f"czo_decompress_field_{s}").argtypes = [
                 C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                 C.c_void_p, C.c_size_t, C.c_void_p]
+        L.czo_set_threads.argtypes = [C.c_int]
+        L.czo_get_threads.restype = C.c_int
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """OpenMP threads used by the field drivers."""
+    lib().czo_set_threads(int(n))
+    return lib().czo_get_threads()
 
 
 def build(ref=True):

@@ -1,0 +1,121 @@
+"""GPU parity: the HIP compressor against the oracle, bit for bit.
+
+Compress: every record payload (mask + survivors, after zerobits/shuffle)
+must equal the oracle's payload for the same block.  Decompress: the decoded
+field must equal the oracle's decoded field bit for bit.
+"""
+import numpy as np
+import pytest
+
+from cubismz_amd import czw, fields
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (64, 96, 128)  # 2 x 3 x 4 = 24 blocks, non-cubic
+
+
+def _field(torch, seed, shape=SHAPE, kind="bubble", dtype=None):
+    dtype = dtype or torch.float32
+    if kind == "bubble":
+        nb = max(4, int(200 * np.prod(shape) / 512 ** 3 * 8))
+        return fields.bubble_field(shape, seed=seed, nbubbles=nb, dtype=dtype)
+    return fields.bumps_field(shape, seed=seed, dtype=dtype)
+
+
+def _payloads(records, offsets):
+    return czw.records_to_payloads(records.cpu().numpy(), offsets.cpu().numpy())
+
+
+@pytest.mark.parametrize("wtype", [1, 2, 3])
+@pytest.mark.parametrize("zerobits,shuffle", [(0, False), (4, False), (0, True), (4, True)])
+@pytest.mark.parametrize("eps", [1e-2, 1e-3, 1e-4])
+def test_field_compress_bitexact(torch_gpu, oracle, wtype, zerobits, shuffle, eps):
+    torch = torch_gpu
+    print(f"[case] field {SHAPE} wtype={wtype} zb={zerobits} sh={shuffle} eps={eps}", flush=True)
+    f = _field(torch, seed=11 + wtype)
+    rec, off = czw.compress_field(f.cuda(), eps, wtype, zerobits, shuffle)
+    torch.cuda.synchronize()
+    got = _payloads(rec, off)
+    slots, nbytes = oracle.compress_field(f.numpy(), eps, wtype, zerobits, shuffle)
+    assert len(got) == len(nbytes)
+    for b, p in enumerate(got):
+        assert len(p) == nbytes[b], f"block {b}: size {len(p)} vs {nbytes[b]}"
+        assert p == slots[b, :nbytes[b]].tobytes(), f"block {b}: payload differs"
+    # records are contiguous and framed with their sizes
+    o = off.cpu().numpy()
+    assert o[0] == 0 and np.all(np.diff(o) == 4 + nbytes.astype(np.int64))
+
+
+@pytest.mark.parametrize("wtype", [1, 2, 3])
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_field_roundtrip_bitexact(torch_gpu, oracle, wtype, shuffle):
+    torch = torch_gpu
+    print(f"[case] roundtrip {SHAPE} wtype={wtype} sh={shuffle}", flush=True)
+    f = _field(torch, seed=21 + wtype, kind="bumps")
+    for eps, zb in ((1e-3, 4), (1e-4, 0)):
+        rec, off = czw.compress_field(f.cuda(), eps, wtype, zb, shuffle)
+        out = czw.decompress_field(rec, off, SHAPE, torch.float32, wtype, shuffle).cpu().numpy()
+        slots, nbytes = oracle.compress_field(f.numpy(), eps, wtype, zb, shuffle)
+        ref = oracle.decompress_field(slots, nbytes, SHAPE, wtype, "f32", shuffle)
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("wtype", [1, 3])
+def test_blocks_layout(torch_gpu, oracle, wtype):
+    torch = torch_gpu
+    print(f"[case] blocks wtype={wtype}", flush=True)
+    rng = np.random.default_rng(5)
+    blocks = (rng.standard_normal((6, 32, 32, 32)) * np.logspace(-3, 0, 32)[None, :, None, None]).astype(np.float32)
+    rec, off = czw.compress_blocks(torch.from_numpy(blocks).cuda(), 1e-2, wtype, 0, False)
+    got = _payloads(rec, off)
+    for b in range(6):
+        assert got[b] == oracle.compress(blocks[b], 1e-2, wtype).tobytes()
+    out = czw.decompress_blocks(rec, off, 6, torch.float32, wtype).cpu().numpy()
+    for b in range(6):
+        ref = oracle.decompress(got[b], wtype, "f32")
+        assert np.array_equal(out[b].view(np.uint32), ref.view(np.uint32))
+
+
+def test_eps_zero_keeps_every_nonzero(torch_gpu, oracle):
+    """eps = 0: every non-zero coefficient survives and decode equals iwt(fwt(x))."""
+    torch = torch_gpu
+    rng = np.random.default_rng(9)
+    blocks = rng.standard_normal((3, 32, 32, 32)).astype(np.float32)
+    rec, off = czw.compress_blocks(torch.from_numpy(blocks).cuda(), 0.0, 3)
+    got = _payloads(rec, off)
+    out = czw.decompress_blocks(rec, off, 3, torch.float32, 3).cpu().numpy()
+    for b in range(3):
+        coef = oracle.fwt(blocks[b], 3)
+        assert len(got[b]) == 4096 + 4 * int(np.count_nonzero(coef))
+        assert np.array_equal(out[b], oracle.iwt(coef, 3))
+
+
+def test_corrupt_record_sets_error(torch_gpu):
+    torch = torch_gpu
+    f = _field(torch, seed=3)
+    rec, off = czw.compress_field(f.cuda(), 1e-3, 3)
+    bad = rec.clone()
+    o = int(off[1].item())
+    bad[o:o + 4] = torch.tensor([1, 2, 0, 0], dtype=torch.uint8)  # nbytes not 4096 + 4k
+    with pytest.raises(czw.CZWError):
+        czw.decompress_field(bad, off, SHAPE, torch.float32, 3)
+    bad = rec.clone()
+    bad[o + 4 + 100] ^= 0xFF  # flip mask bits: popcount no longer matches
+    with pytest.raises(czw.CZWError):
+        czw.decompress_field(bad, off, SHAPE, torch.float32, 3)
+
+
+@pytest.mark.parametrize("wtype", [1, 2, 3])
+def test_against_compiled_reference(torch_gpu, oracle, wtype):
+    """Where oracle/_ref was built, the GPU payloads equal the reference's own."""
+    torch = torch_gpu
+    if not oracle.ref_available("f32", 4, True):
+        pytest.skip("oracle/_ref not built")
+    f = _field(torch, seed=31)
+    rec, off = czw.compress_field(f.cuda(), 1e-3, wtype, 4, True)
+    got = _payloads(rec, off)
+    fn = f.numpy()
+    for b in (0, 7, 23):
+        ix, iy, iz = b % 2, (b // 2) % 3, b // 6
+        blk = np.ascontiguousarray(fn[32 * ix:32 * ix + 32, 32 * iy:32 * iy + 32, 32 * iz:32 * iz + 32].transpose(2, 1, 0))
+        assert got[b] == oracle.ref_compress(blk, 1e-3, wtype, 4, True).tobytes()
