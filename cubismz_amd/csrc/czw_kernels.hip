@@ -405,7 +405,7 @@ __global__ __launch_bounds__(256) void k_pack_records(const uint8_t *__restrict_
 // (WaveletCompressor.cpp:232-259): mask, reshuffle, load (stream order,
 // zeros for dropped coefficients; FullWaveletTransform.h:100-150), iwt.
 template <int WT, int LAYOUT>
-__global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict__ records,
+__global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict__ records, uint64_t rec_bytes,
                                                        const uint64_t *__restrict__ offsets, int shuffle,
                                                        float *__restrict__ dst, FieldGeom g,
                                                        uint32_t *__restrict__ err)
@@ -415,9 +415,14 @@ __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict
     __shared__ uint32_t wsum[NT / 64];
     const int t = threadIdx.x;
     const int b = blockIdx.x;
-    const uint8_t *rec = records + offsets[b];
+    const uint64_t ob = offsets[b];
+    if ((ob & 3u) || ob + 4 + MASK_BYTES > rec_bytes) {
+        if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_NBYTES);
+        return;
+    }
+    const uint8_t *rec = records + ob;
     const uint32_t nb = *reinterpret_cast<const uint32_t *>(rec);
-    if (nb < MASK_BYTES || ((nb - MASK_BYTES) & 3u) || nb > SLOT32) {
+    if (nb < MASK_BYTES || ((nb - MASK_BYTES) & 3u) || nb > SLOT32 || ob + 4 + nb > rec_bytes) {
         if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_NBYTES);
         return;
     }
@@ -541,16 +546,16 @@ static int compress_f32(const float *src, int nblocks, FieldGeom g, float eps, i
 }
 
 template <int LAYOUT>
-static int decompress_f32(const void *records, const uint64_t *offsets, int nblocks, FieldGeom g, int wtype,
-                          int shuffle, float *dst, uint32_t *err, hipStream_t s)
+static int decompress_f32(const void *records, size_t rec_bytes, const uint64_t *offsets, int nblocks, FieldGeom g,
+                          int wtype, int shuffle, float *dst, uint32_t *err, hipStream_t s)
 {
     const int wt = wt_index(wtype);
     if (wt < 0 || nblocks <= 0 || !records || !offsets || !dst || !err) return CZW_EINVAL;
     const uint8_t *rec = static_cast<const uint8_t *>(records);
     const dim3 grid(nblocks), blk(NT);
-    if (wt == 1) hipLaunchKernelGGL((k_decompress_f32<1, LAYOUT>), grid, blk, 0, s, rec, offsets, shuffle, dst, g, err);
-    if (wt == 2) hipLaunchKernelGGL((k_decompress_f32<2, LAYOUT>), grid, blk, 0, s, rec, offsets, shuffle, dst, g, err);
-    if (wt == 3) hipLaunchKernelGGL((k_decompress_f32<3, LAYOUT>), grid, blk, 0, s, rec, offsets, shuffle, dst, g, err);
+    if (wt == 1) hipLaunchKernelGGL((k_decompress_f32<1, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, shuffle, dst, g, err);
+    if (wt == 2) hipLaunchKernelGGL((k_decompress_f32<2, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, shuffle, dst, g, err);
+    if (wt == 3) hipLaunchKernelGGL((k_decompress_f32<3, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, shuffle, dst, g, err);
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
 
@@ -624,25 +629,25 @@ int czw_compress_blocks(const void *blocks, int nblocks, int prec, float eps, in
     return prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
 }
 
-int czw_decompress_field(const void *records, const uint64_t *offsets, int nx, int ny, int nz, int prec, int wtype,
-                         int shuffle, void *field, uint32_t *err, hipStream_t stream)
+int czw_decompress_field(const void *records, size_t records_bytes, const uint64_t *offsets, int nx, int ny, int nz,
+                         int prec, int wtype, int shuffle, void *field, uint32_t *err, hipStream_t stream)
 {
     if (!field_ok(nx, ny, nz)) return CZW_EINVAL;
     const int nb = (nx / 32) * (ny / 32) * (nz / 32);
     const FieldGeom g{nx, ny, nz};
     if (prec == 4)
-        return decompress_f32<LAYOUT_FIELD>(records, offsets, nb, g, wtype, shuffle, static_cast<float *>(field), err,
-                                            stream);
+        return decompress_f32<LAYOUT_FIELD>(records, records_bytes, offsets, nb, g, wtype, shuffle,
+                                            static_cast<float *>(field), err, stream);
     return prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
 }
 
-int czw_decompress_blocks(const void *records, const uint64_t *offsets, int nblocks, int prec, int wtype, int shuffle,
-                          void *blocks, uint32_t *err, hipStream_t stream)
+int czw_decompress_blocks(const void *records, size_t records_bytes, const uint64_t *offsets, int nblocks, int prec,
+                          int wtype, int shuffle, void *blocks, uint32_t *err, hipStream_t stream)
 {
     const FieldGeom g{0, 0, 0};
     if (prec == 4)
-        return decompress_f32<LAYOUT_BLOCKS>(records, offsets, nblocks, g, wtype, shuffle, static_cast<float *>(blocks),
-                                             err, stream);
+        return decompress_f32<LAYOUT_BLOCKS>(records, records_bytes, offsets, nblocks, g, wtype, shuffle,
+                                             static_cast<float *>(blocks), err, stream);
     return prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
 }
 

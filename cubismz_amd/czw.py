@@ -62,9 +62,9 @@ def load_library():
     L.czw_compress_blocks.restype = i
     L.czw_compress_blocks.argtypes = [vp, i, i, f, i, i, i, vp, vp, vp, vp]
     L.czw_decompress_field.restype = i
-    L.czw_decompress_field.argtypes = [vp, vp, i, i, i, i, i, i, vp, vp, vp]
+    L.czw_decompress_field.argtypes = [vp, sz, vp, i, i, i, i, i, i, vp, vp, vp]
     L.czw_decompress_blocks.restype = i
-    L.czw_decompress_blocks.argtypes = [vp, vp, i, i, i, i, vp, vp, vp]
+    L.czw_decompress_blocks.argtypes = [vp, sz, vp, i, i, i, i, vp, vp, vp]
     _lib = L
     return L
 
@@ -198,7 +198,8 @@ def decompress_field(records, offsets, shape, dtype, wtype=3, shuffle=False, out
     if out is None:
         out = torch.empty((nx, ny, nz), dtype=dtype, device=records.device)
     err = _err_buffer(torch, records.device, err)
-    rc = L.czw_decompress_field(C.c_void_p(records.data_ptr()), C.c_void_p(offsets.data_ptr()), nx, ny, nz, prec,
+    rc = L.czw_decompress_field(C.c_void_p(records.data_ptr()), records.numel() * records.element_size(),
+                                C.c_void_p(offsets.data_ptr()), nx, ny, nz, prec,
                                 int(wtype), int(bool(shuffle)), C.c_void_p(out.data_ptr()),
                                 C.c_void_p(err.data_ptr()), _stream(torch))
     _check(rc, "czw_decompress_field")
@@ -215,7 +216,8 @@ def decompress_blocks(records, offsets, nblocks, dtype, wtype=3, shuffle=False, 
     if out is None:
         out = torch.empty((nblocks, BLOCK, BLOCK, BLOCK), dtype=dtype, device=records.device)
     err = _err_buffer(torch, records.device, err)
-    rc = L.czw_decompress_blocks(C.c_void_p(records.data_ptr()), C.c_void_p(offsets.data_ptr()), int(nblocks), prec,
+    rc = L.czw_decompress_blocks(C.c_void_p(records.data_ptr()), records.numel() * records.element_size(),
+                                 C.c_void_p(offsets.data_ptr()), int(nblocks), prec,
                                  int(wtype), int(bool(shuffle)), C.c_void_p(out.data_ptr()),
                                  C.c_void_p(err.data_ptr()), _stream(torch))
     _check(rc, "czw_decompress_blocks")

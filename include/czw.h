@@ -64,7 +64,7 @@ enum {
 
 /* Device error bits written by the decompressors into *err (OR-ed). */
 enum {
-    CZW_ERR_NBYTES = 1,   /* record size not 4096 + k*prec within bounds */
+    CZW_ERR_NBYTES = 1,   /* record size/offset invalid or outside the records buffer */
     CZW_ERR_POPCOUNT = 2  /* mask popcount differs from the survivor count */
 };
 
@@ -103,15 +103,17 @@ int czw_compress_blocks(const void *blocks, int nblocks, int prec, float eps, in
 /* Replaces WaveletCompressorGeneric::decompress (WaveletCompressor.cpp:232-259)
  * plus copy_to (WaveletCompressor.h:73-82) for every record, writing an
  * HDF5-ordered field.  offsets[b] points at record b's int32 size word; the
- * records need not be contiguous.  *err (device) is OR-ed with CZW_ERR_* for
- * corrupt records, whose blocks are left unwritten. */
-int czw_decompress_field(const void *records, const uint64_t *offsets, int nx, int ny, int nz,
-                         int prec, int wtype, int shuffle, void *field, uint32_t *err,
+ * records need not be contiguous, but every record must lie inside
+ * [records, records + records_bytes).  *err (device) is OR-ed with CZW_ERR_*
+ * for corrupt or out-of-range records, whose blocks are left unwritten. */
+int czw_decompress_field(const void *records, size_t records_bytes, const uint64_t *offsets, int nx,
+                         int ny, int nz, int prec, int wtype, int shuffle, void *field, uint32_t *err,
                          hipStream_t stream);
 
 /* Same as czw_decompress_field writing blocks [nb][z][y][x]. */
-int czw_decompress_blocks(const void *records, const uint64_t *offsets, int nblocks, int prec,
-                          int wtype, int shuffle, void *blocks, uint32_t *err, hipStream_t stream);
+int czw_decompress_blocks(const void *records, size_t records_bytes, const uint64_t *offsets,
+                          int nblocks, int prec, int wtype, int shuffle, void *blocks, uint32_t *err,
+                          hipStream_t stream);
 
 #ifdef __cplusplus
 }
