@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "../../include/czw.h"
 #include "czw_wavelet.hpp"
 
@@ -35,7 +37,8 @@ constexpr int MASK_BYTES = 4096;
 constexpr int RP = 33;
 constexpr int SP = 32 * 33 + 1;
 constexpr int LDS_WORDS = 32 * SP;
-constexpr int NT = 1024;
+constexpr int NT = 512;            // threads per block workgroup (8 wave64)
+constexpr int NROW = 1024 / NT;    // level-32 rows (and stream segments) per thread
 constexpr size_t SLOT32 = MASK_BYTES + 4 * (size_t)NB3;
 
 enum { LAYOUT_BLOCKS = 0, LAYOUT_FIELD = 1 };
@@ -65,8 +68,9 @@ CZW_DI uint32_t wave_incl_scan(uint32_t x, int lane)
     return x;
 }
 
-// Block-wide exclusive scan of one value per thread (1024 threads).
-// Returns the exclusive prefix; *total receives the sum.
+// Block-wide exclusive scan of one value per thread.  Returns the exclusive
+// prefix; *total receives the sum.  Ends with the wave totals read, so the
+// caller must barrier before wsum is written again.
 CZW_DI uint32_t block_excl_scan(uint32_t v, uint32_t *wsum, uint32_t *total)
 {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -85,32 +89,33 @@ CZW_DI uint32_t block_excl_scan(uint32_t v, uint32_t *wsum, uint32_t *total)
 }
 
 // ---------------------------------------------------------------------------
-// Stream segment of thread t: stream indices [32t, 32t+32).  Stream order
-// (FullWaveletTransform.h:59-96,163-176): [0,512) is the [0,8)^3 corner as
-// codes 0..7 of boxes of h=4 (code 0 = the untouched 4^3 base), [512,4096)
-// codes 1..7 of h=8, [4096,32768) codes 1..7 of h=16.  Inside a code box
-// (origin h*(code&1, code>>1&1, code>>2&1) on (a2,a1,a0)) the order is
-// a0, a1, a2 with a2 fastest, so a segment is 32/h runs of h consecutive a2.
-// For each run the callback receives the run index q, the LDS address of its
-// first element, the LDS stride along the run, the mask word p>>5 and the
-// bit shift of its first element.
+// Stream segment g covers stream indices [32g, 32g+32) (g = t + NT*k for the
+// k-th segment of thread t).  Stream order (FullWaveletTransform.h:59-96,
+// 163-176): [0,512) is the [0,8)^3 corner as codes 0..7 of boxes of h=4
+// (code 0 = the untouched 4^3 base), [512,4096) codes 1..7 of h=8,
+// [4096,32768) codes 1..7 of h=16.  Inside a code box (origin
+// h*(code&1, code>>1&1, code>>2&1) on (a2,a1,a0)) the order is a0, a1, a2
+// with a2 fastest, so a segment is 32/h runs of h consecutive a2.  For each
+// run the callback receives the run index q, the run length h, the LDS address
+// of its first element, the LDS stride along the run, the mask word p>>5 and
+// the bit shift of its first element.
 template <typename F>
-CZW_DI void for_each_run(int t, F &&f)
+CZW_DI void for_each_run(int g, F &&f)
 {
-    if (t < 16) {  // h = 4: coarse base + level-8 details, O = (a0,a1,a2)
+    if (g < 16) {  // h = 4: coarse base + level-8 details, O = (a0,a1,a2)
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const int s = 32 * t + 4 * q;
+            const int s = 32 * g + 4 * q;
             const int code = s >> 6, k = s & 63;
             const int a0 = 4 * ((code >> 2) & 1) + (k >> 4);
             const int a1 = 4 * ((code >> 1) & 1) + ((k >> 2) & 3);
             const int o2 = 4 * (code & 1);
             f(q, 4, a0 * SP + a1 * RP + o2, 1, a1 + 32 * a0, o2);
         }
-    } else if (t < 128) {  // h = 8: level-16 details, O.x=a0, O.z=a1, O.y=a2
+    } else if (g < 128) {  // h = 8: level-16 details, O.x=a0, O.z=a1, O.y=a2
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int s = 32 * t + 8 * q;
+            const int s = 32 * g + 8 * q;
             const int code = s >> 9, k = s & 511;
             const int a0 = 8 * ((code >> 2) & 1) + (k >> 6);
             const int a1 = 8 * ((code >> 1) & 1) + ((k >> 3) & 7);
@@ -120,7 +125,7 @@ CZW_DI void for_each_run(int t, F &&f)
     } else {  // h = 16: level-32 details, O.y=a0, O.x=a1, O.z=a2
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const int s = 32 * t + 16 * q;
+            const int s = 32 * g + 16 * q;
             const int code = s >> 12, k = s & 4095;
             const int a0 = 16 * ((code >> 2) & 1) + (k >> 8);
             const int a1 = 16 * ((code >> 1) & 1) + ((k >> 4) & 15);
@@ -131,39 +136,45 @@ CZW_DI void for_each_run(int t, F &&f)
 }
 
 // ---------------------------------------------------------------------------
-// Forward transform of the block held in A (all 3 levels), fused where one
-// level's last pass and the next level's first pass share an axis.
+// Forward level-32 y- and z-passes (the x-pass runs on the rows as loaded),
+// with the level-16 z-pass fused into the level-32 z-pass.
 template <int WT>
-CZW_DI void fwt_rest_f32(float *A, int t)
+CZW_DI void fwt32_yz_f32(float *A, int t)
 {
-    float r[32];
-    // level 32, y-pass: thread (z, x)
-    {
-        const int z = t >> 5, x = t & 31;
+#pragma unroll
+    for (int k = 0; k < NROW; ++k) {  // y-pass: row (z, x)
+        const int q = t + NT * k, z = q >> 5, x = q & 31;
+        float r[32];
         lds_ld<32>(A, z * SP + x, RP, r);
         row<32, true, WT>(r);
         lds_st<32>(A, z * SP + x, RP, r);
     }
     __syncthreads();
-    // level 32 z-pass, then level 16 z-pass: thread (y, x)
-    {
-        const int y = t >> 5, x = t & 31;
+#pragma unroll
+    for (int k = 0; k < NROW; ++k) {  // z-pass: row (y, x); then level 16 on y, x < 16
+        const int q = t + NT * k, y = q >> 5, x = q & 31;
+        float r[32];
         lds_ld<32>(A, y * RP + x, SP, r);
         row<32, true, WT>(r);
         if (y < 16 && x < 16) row<16, true, WT>(r);
         lds_st<32>(A, y * RP + x, SP, r);
     }
     __syncthreads();
-    // level 16 x-pass: thread (z, y) in [0,16)^2
-    if (t < 256) {
+}
+
+// Remaining forward passes: level 16 (x, y) and level 8 (y fused, z, x).
+template <int WT>
+CZW_DI void fwt16_8_f32(float *A, int t)
+{
+    float r[16];
+    if (t < 256) {  // level 16 x-pass: row (z, y) in [0,16)^2
         const int z = t >> 4, y = t & 15;
         lds_ld<16>(A, z * SP + y * RP, 1, r);
         row<16, true, WT>(r);
         lds_st<16>(A, z * SP + y * RP, 1, r);
     }
     __syncthreads();
-    // level 16 y-pass, then level 8 y-pass: thread (z, x) in [0,16)^2
-    if (t < 256) {
+    if (t < 256) {  // level 16 y-pass, then level 8 y-pass: row (z, x)
         const int z = t >> 4, x = t & 15;
         lds_ld<16>(A, z * SP + x, RP, r);
         row<16, true, WT>(r);
@@ -171,16 +182,14 @@ CZW_DI void fwt_rest_f32(float *A, int t)
         lds_st<16>(A, z * SP + x, RP, r);
     }
     __syncthreads();
-    // level 8 z-pass: thread (y, x) in [0,8)^2
-    if (t < 64) {
+    if (t < 64) {  // level 8 z-pass: row (y, x) in [0,8)^2
         const int y = t >> 3, x = t & 7;
         lds_ld<8>(A, y * RP + x, SP, r);
         row<8, true, WT>(r);
         lds_st<8>(A, y * RP + x, SP, r);
     }
     __syncthreads();
-    // level 8 x-pass: thread (z, y) in [0,8)^2
-    if (t < 64) {
+    if (t < 64) {  // level 8 x-pass: row (z, y)
         const int z = t >> 3, y = t & 7;
         lds_ld<8>(A, z * SP + y * RP, 1, r);
         row<8, true, WT>(r);
@@ -189,57 +198,58 @@ CZW_DI void fwt_rest_f32(float *A, int t)
     __syncthreads();
 }
 
-// Inverse transform of the coefficients held in A, except for the final
-// level-32 x-pass which the caller fuses with the store.
+// Inverse passes of all levels except the final level-32 x-pass, which the
+// caller fuses with the store: level 8 (x, z, y), level 16 (y fused, x, z),
+// level 32 (z fused, y).
 template <int WT>
 CZW_DI void iwt_most_f32(float *A, int t)
 {
-    float r[32];
-    // level 8 x-pass
-    if (t < 64) {
-        const int z = t >> 3, y = t & 7;
-        lds_ld<8>(A, z * SP + y * RP, 1, r);
-        row<8, false, WT>(r);
-        lds_st<8>(A, z * SP + y * RP, 1, r);
-    }
-    __syncthreads();
-    // level 8 z-pass
-    if (t < 64) {
-        const int y = t >> 3, x = t & 7;
-        lds_ld<8>(A, y * RP + x, SP, r);
-        row<8, false, WT>(r);
-        lds_st<8>(A, y * RP + x, SP, r);
-    }
-    __syncthreads();
-    // level 8 y-pass, then level 16 y-pass
-    if (t < 256) {
-        const int z = t >> 4, x = t & 15;
-        lds_ld<16>(A, z * SP + x, RP, r);
-        if (z < 8 && x < 8) row<8, false, WT>(r);
-        row<16, false, WT>(r);
-        lds_st<16>(A, z * SP + x, RP, r);
-    }
-    __syncthreads();
-    // level 16 x-pass
-    if (t < 256) {
-        const int z = t >> 4, y = t & 15;
-        lds_ld<16>(A, z * SP + y * RP, 1, r);
-        row<16, false, WT>(r);
-        lds_st<16>(A, z * SP + y * RP, 1, r);
-    }
-    __syncthreads();
-    // level 16 z-pass, then level 32 z-pass
     {
-        const int y = t >> 5, x = t & 31;
+        float r[16];
+        if (t < 64) {  // level 8 x-pass
+            const int z = t >> 3, y = t & 7;
+            lds_ld<8>(A, z * SP + y * RP, 1, r);
+            row<8, false, WT>(r);
+            lds_st<8>(A, z * SP + y * RP, 1, r);
+        }
+        __syncthreads();
+        if (t < 64) {  // level 8 z-pass
+            const int y = t >> 3, x = t & 7;
+            lds_ld<8>(A, y * RP + x, SP, r);
+            row<8, false, WT>(r);
+            lds_st<8>(A, y * RP + x, SP, r);
+        }
+        __syncthreads();
+        if (t < 256) {  // level 8 y-pass, then level 16 y-pass
+            const int z = t >> 4, x = t & 15;
+            lds_ld<16>(A, z * SP + x, RP, r);
+            if (z < 8 && x < 8) row<8, false, WT>(r);
+            row<16, false, WT>(r);
+            lds_st<16>(A, z * SP + x, RP, r);
+        }
+        __syncthreads();
+        if (t < 256) {  // level 16 x-pass
+            const int z = t >> 4, y = t & 15;
+            lds_ld<16>(A, z * SP + y * RP, 1, r);
+            row<16, false, WT>(r);
+            lds_st<16>(A, z * SP + y * RP, 1, r);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < NROW; ++k) {  // level 16 z-pass, then level 32 z-pass
+        const int q = t + NT * k, y = q >> 5, x = q & 31;
+        float r[32];
         lds_ld<32>(A, y * RP + x, SP, r);
         if (y < 16 && x < 16) row<16, false, WT>(r);
         row<32, false, WT>(r);
         lds_st<32>(A, y * RP + x, SP, r);
     }
     __syncthreads();
-    // level 32 y-pass
-    {
-        const int z = t >> 5, x = t & 31;
+#pragma unroll
+    for (int k = 0; k < NROW; ++k) {  // level 32 y-pass
+        const int q = t + NT * k, z = q >> 5, x = q & 31;
+        float r[32];
         lds_ld<32>(A, z * SP + x, RP, r);
         row<32, false, WT>(r);
         lds_st<32>(A, z * SP + x, RP, r);
@@ -251,8 +261,8 @@ struct FieldGeom {
     int nx, ny, nz;
 };
 
-// Global element offset of block b's row start (x = 0) for thread-owned (y, z)
-// in an HDF5-ordered field, and the stride along x.
+// Global element offset of block b's row start (x = 0) for the x-row (y, z)
+// of an HDF5-ordered field, and the stride along x.
 CZW_DI size_t field_row(const FieldGeom &g, int b, int y, int z, size_t *xstride)
 {
     const int bx = g.nx >> 5, by = g.ny >> 5;
@@ -262,127 +272,188 @@ CZW_DI size_t field_row(const FieldGeom &g, int b, int y, int z, size_t *xstride
 }
 
 // ---------------------------------------------------------------------------
-// Compressor, binary32.  Writes block b's payload to slots + b*SLOT32 and its
-// size to nbytes[b].  Mirrors WaveletCompressorGeneric::compress
+// x-row q (0..1023) of a block: in the block layout q = (z, y) with y
+// fastest (16-byte loads along x); in the HDF5 field layout q = (y, z) with z
+// fastest, so that each 4-byte load of a wave reads two runs of 32
+// consecutive Z of the field.
+template <int LAYOUT>
+CZW_DI int xrow_lds_base(int q)
+{
+    if constexpr (LAYOUT == LAYOUT_BLOCKS) return (q >> 5) * SP + (q & 31) * RP;
+    else return (q & 31) * SP + (q >> 5) * RP;
+}
+
+template <int LAYOUT>
+CZW_DI void load_xrow_f32(const float *__restrict__ src, const FieldGeom &g, int b, int q, float *r)
+{
+    if constexpr (LAYOUT == LAYOUT_BLOCKS) {
+        const float4 *p = reinterpret_cast<const float4 *>(src + (size_t)b * NB3 + (size_t)q * 32);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const float4 v = p[i];
+            r[4 * i] = v.x;
+            r[4 * i + 1] = v.y;
+            r[4 * i + 2] = v.z;
+            r[4 * i + 3] = v.w;
+        }
+    } else {
+        size_t xs;
+        const float *p = src + field_row(g, b, q >> 5, q & 31, &xs);
+#pragma unroll
+        for (int x = 0; x < 32; ++x) r[x] = p[x * xs];
+    }
+}
+
+template <int LAYOUT>
+CZW_DI void store_xrow_f32(float *__restrict__ dst, const FieldGeom &g, int b, int q, const float *r)
+{
+    if constexpr (LAYOUT == LAYOUT_BLOCKS) {
+        float4 *p = reinterpret_cast<float4 *>(dst + (size_t)b * NB3 + (size_t)q * 32);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) p[i] = make_float4(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
+    } else {
+        size_t xs;
+        float *p = dst + field_row(g, b, q >> 5, q & 31, &xs);
+#pragma unroll
+        for (int x = 0; x < 32; ++x) p[x * xs] = r[x];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Compressor, binary32.  Mirrors WaveletCompressorGeneric::compress
 // (WaveletCompressor.cpp:153-182): fwt, threshold + mask, zerobits, shuffle.
+// Persistent: workgroup w handles blocks w, w+G, w+2G, ... (G = gridDim.x)
+// and loads block b+G into registers while block b is transformed, so the
+// HBM reads of one block overlap the arithmetic of the previous one.  Block
+// b's payload goes to slots + b*SLOT32 and its size to nbytes[b].
 template <int WT, int LAYOUT>
-__global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ src, FieldGeom g, float eps,
-                                                     uint32_t zkeep, int shuffle, uint8_t *__restrict__ slots,
-                                                     uint32_t *__restrict__ nbytes)
+__global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ src, FieldGeom g, int nblocks,
+                                                     float eps, uint32_t zkeep, int shuffle,
+                                                     uint8_t *__restrict__ slots, uint32_t *__restrict__ nbytes)
 {
     __shared__ float A[LDS_WORDS];
-    __shared__ uint32_t M[NT];
+    __shared__ uint32_t M[1024];
     __shared__ uint32_t wsum[NT / 64];
     const int t = threadIdx.x;
-    const int b = blockIdx.x;
-    M[t] = 0u;
-    {
-        float r[32];
-        int z, y;
-        if constexpr (LAYOUT == LAYOUT_BLOCKS) {
-            z = t >> 5;
-            y = t & 31;
-            const float4 *p = reinterpret_cast<const float4 *>(src + (size_t)b * NB3 + (size_t)(z * 32 + y) * 32);
+    const int G = gridDim.x;
+    float nx[NROW][32];
+    int b = blockIdx.x;
+    if (b < nblocks) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const float4 v = p[q];
-                r[4 * q] = v.x;
-                r[4 * q + 1] = v.y;
-                r[4 * q + 2] = v.z;
-                r[4 * q + 3] = v.w;
-            }
-        } else {
-            y = t >> 5;
-            z = t & 31;
-            size_t xs;
-            const float *p = src + field_row(g, b, y, z, &xs);
-#pragma unroll
-            for (int x = 0; x < 32; ++x) r[x] = p[x * xs];
-        }
-        row<32, true, WT>(r);  // level 32, x-pass
-        lds_st<32>(A, z * SP + y * RP, 1, r);
+        for (int k = 0; k < NROW; ++k) load_xrow_f32<LAYOUT>(src, g, b, t + NT * k, nx[k]);
     }
-    __syncthreads();
-    fwt_rest_f32<WT>(A, t);
-
-    // threshold: keep bits of this thread's stream segment (base 4^3 always kept)
-    float v[32];
-    uint32_t keep = 0;
-    for_each_run(t, [&](int q, int h, int addr, int stride, int word, int shift) {
-        uint32_t bits = 0;
+    for (; b < nblocks; b += G) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            if (i < h) {
-                const float c = A[addr + i * stride];
-                v[q * h + i] = c;
-                const bool k = (t < 2) || (fabsf(c) > eps);
-                bits |= (uint32_t)k << i;
+        for (int k = 0; k < NROW; ++k) {
+            M[t + NT * k] = 0u;
+            float r[32];
+#pragma unroll
+            for (int i = 0; i < 32; ++i) r[i] = nx[k][i];
+            row<32, true, WT>(r);  // level 32, x-pass
+            lds_st<32>(A, xrow_lds_base<LAYOUT>(t + NT * k), 1, r);
+        }
+        __syncthreads();
+        if (b + G < nblocks) {  // prefetch the next block while this one is transformed
+#pragma unroll
+            for (int k = 0; k < NROW; ++k) load_xrow_f32<LAYOUT>(src, g, b + G, t + NT * k, nx[k]);
+        }
+        fwt32_yz_f32<WT>(A, t);
+        fwt16_8_f32<WT>(A, t);
+
+        // threshold: keep bits of this thread's stream segments (base 4^3 always kept)
+        float v[NROW][32];
+        uint32_t keep[NROW];
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) {
+            const int sg = t + NT * k;
+            keep[k] = 0;
+            for_each_run(sg, [&](int q, int h, int addr, int stride, int word, int shift) {
+                uint32_t bits = 0;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    if (i < h) {
+                        const float c = A[addr + i * stride];
+                        v[k][q * h + i] = c;
+                        const bool kp = (sg < 2) || (fabsf(c) > eps);
+                        bits |= (uint32_t)kp << i;
+                    }
+                }
+                keep[k] |= bits << (q * h);
+                if (bits) atomicOr(&M[word], bits << shift);
+            });
+        }
+        // segment counts: low 16 bits = segment t, high 16 bits = segment t+NT
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan((uint32_t)__popc(keep[0]) | ((uint32_t)__popc(keep[1]) << 16), wsum, &tot);
+        const uint32_t S = (tot & 0xffffu) + (tot >> 16);
+        uint32_t first[NROW] = {ex & 0xffffu, (tot & 0xffffu) + (ex >> 16)};
+        __syncthreads();  // every thread has read A: reuse it as the survivor stage
+
+        uint32_t *stage = reinterpret_cast<uint32_t *>(A);
+        uint8_t *stage8 = reinterpret_cast<uint8_t *>(A);
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) {
+            uint32_t idx = first[k];
+#pragma unroll
+            for (int j = 0; j < 32; ++j) {
+                if ((keep[k] >> j) & 1u) {
+                    const uint32_t w = __float_as_uint(v[k][j]) & zkeep;
+                    if (shuffle) {
+                        stage8[idx] = (uint8_t)w;
+                        stage8[S + idx] = (uint8_t)(w >> 8);
+                        stage8[2 * S + idx] = (uint8_t)(w >> 16);
+                        stage8[3 * S + idx] = (uint8_t)(w >> 24);
+                    } else {
+                        stage[idx] = w;
+                    }
+                    ++idx;
+                }
             }
         }
-        keep |= bits << (q * h);
-        if (bits) atomicOr(&M[word], bits << shift);
-    });
-    uint32_t S;
-    const uint32_t first = block_excl_scan(__popc(keep), wsum, &S);
-    __syncthreads();  // every thread has read A: reuse it as the survivor stage
-
-    uint32_t *stage = reinterpret_cast<uint32_t *>(A);
-    uint8_t *stage8 = reinterpret_cast<uint8_t *>(A);
-    uint32_t idx = first;
+        __syncthreads();
+        uint8_t *slot = slots + (size_t)b * SLOT32;
 #pragma unroll
-    for (int j = 0; j < 32; ++j) {
-        if ((keep >> j) & 1u) {
-            const uint32_t w = __float_as_uint(v[j]) & zkeep;
-            if (shuffle) {
-                stage8[idx] = (uint8_t)w;
-                stage8[S + idx] = (uint8_t)(w >> 8);
-                stage8[2 * S + idx] = (uint8_t)(w >> 16);
-                stage8[3 * S + idx] = (uint8_t)(w >> 24);
-            } else {
-                stage[idx] = w;
-            }
-            ++idx;
-        }
+        for (int k = 0; k < NROW; ++k) reinterpret_cast<uint32_t *>(slot)[t + NT * k] = M[t + NT * k];
+        uint32_t *dst = reinterpret_cast<uint32_t *>(slot + MASK_BYTES);
+        for (uint32_t i = t; i < S; i += NT) dst[i] = stage[i];
+        if (t == 0) nbytes[b] = MASK_BYTES + 4u * S;
+        __syncthreads();  // stage and M are read before the next block reuses them
     }
-    __syncthreads();
-    uint8_t *slot = slots + (size_t)b * SLOT32;
-    reinterpret_cast<uint32_t *>(slot)[t] = M[t];
-    uint32_t *dst = reinterpret_cast<uint32_t *>(slot + MASK_BYTES);
-    for (uint32_t i = t; i < S; i += NT) dst[i] = stage[i];
-    if (t == 0) nbytes[b] = MASK_BYTES + 4u * S;
 }
 
 // Exclusive scan of record sizes (4 + nbytes[b]) into offsets[0..n]; one
-// workgroup walking the blocks in tiles of 1024.
-__global__ __launch_bounds__(NT) void k_scan_records(const uint32_t *__restrict__ nbytes, int n,
-                                                     uint64_t *__restrict__ offsets)
+// workgroup of 1024 threads, each owning a contiguous run of ceil(n/1024)
+// blocks so that every load is issued before the scan.
+__global__ __launch_bounds__(1024) void k_scan_records(const uint32_t *__restrict__ nbytes, int n,
+                                                       uint64_t *__restrict__ offsets)
 {
-    __shared__ uint64_t wsum[NT / 64];
+    __shared__ uint64_t wsum[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    uint64_t carry = 0;
-    for (int base = 0; base < n; base += NT) {
-        const int i = base + t;
-        const uint64_t v = (i < n) ? (uint64_t)nbytes[i] + 4u : 0u;
-        uint64_t inc = v;
+    const int per = (n + 1023) / 1024;
+    const int i0 = min(n, t * per), i1 = min(n, i0 + per);
+    uint64_t v = 0;
+    for (int i = i0; i < i1; ++i) v += (uint64_t)nbytes[i] + 4u;
+    uint64_t inc = v;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint64_t y = __shfl_up(inc, d, 64);
-            if (lane >= d) inc += y;
-        }
-        if (lane == 63) wsum[w] = inc;
-        __syncthreads();
-        uint64_t pre = 0, tot = 0;
-#pragma unroll
-        for (int k = 0; k < NT / 64; ++k) {
-            const uint64_t s = wsum[k];
-            pre += (k < w) ? s : 0u;
-            tot += s;
-        }
-        if (i < n) offsets[i] = carry + pre + inc - v;
-        carry += tot;
-        __syncthreads();
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t y = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += y;
     }
-    if (t == 0) offsets[n] = carry;
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint64_t run = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const uint64_t s = wsum[k];
+        run += (k < w) ? s : 0u;
+        tot += s;
+    }
+    run += inc - v;
+    for (int i = i0; i < i1; ++i) {
+        offsets[i] = run;
+        run += (uint64_t)nbytes[i] + 4u;
+    }
+    if (t == 0) offsets[n] = tot;
 }
 
 // Copy each block's payload from its slot into the record stream, prefixed
@@ -394,103 +465,167 @@ __global__ __launch_bounds__(256) void k_pack_records(const uint8_t *__restrict_
 {
     const int b = blockIdx.x, t = threadIdx.x;
     const uint32_t n = nbytes[b];
+    const uint32_t nd = n / 4;
     uint32_t *dst = reinterpret_cast<uint32_t *>(records + offsets[b]);
     const uint32_t *src = reinterpret_cast<const uint32_t *>(slots + (size_t)b * slot_bytes);
     if (t == 0) dst[0] = n;
-    for (uint32_t i = t; i < n / 4; i += 256) dst[1 + i] = src[i];
+    for (uint32_t base = t; base < nd; base += 256 * 4) {
+        uint32_t v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t i = base + 256 * k;
+            v[k] = i < nd ? src[i] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t i = base + 256 * k;
+            if (i < nd) dst[1 + i] = v[k];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Decompressor, binary32.  Mirrors WaveletCompressorGeneric::decompress
 // (WaveletCompressor.cpp:232-259): mask, reshuffle, load (stream order,
 // zeros for dropped coefficients; FullWaveletTransform.h:100-150), iwt.
+// Persistent like the compressor: while block b is decoded, the record of
+// block b+G (size word, mask, first PF*NT survivor words) is already in
+// flight and offsets[b+2G] is being fetched.
+constexpr int PF = 8;
+
+struct RecPrefetch {
+    uint32_t nb, mw[NROW], sv[PF];
+};
+
+CZW_DI void fetch_record(const uint8_t *__restrict__ records, uint64_t rec_bytes, uint64_t ob, int t,
+                         RecPrefetch &r)
+{
+    if ((ob & 3u) || ob + 4 + MASK_BYTES > rec_bytes) {
+        r.nb = 0xffffffffu;  // rejected by the size check
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) r.mw[k] = 0u;
+#pragma unroll
+        for (int k = 0; k < PF; ++k) r.sv[k] = 0u;
+        return;
+    }
+    const uint32_t *p = reinterpret_cast<const uint32_t *>(records + ob);
+    const uint64_t nwords = (rec_bytes - ob) / 4;  // words of the buffer from this record on
+    r.nb = p[0];
+#pragma unroll
+    for (int k = 0; k < NROW; ++k) r.mw[k] = p[1 + t + NT * k];
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+        const uint64_t i = 1 + 1024 + (uint64_t)k * NT + t;
+        r.sv[k] = i < nwords ? p[i] : 0u;
+    }
+}
+
 template <int WT, int LAYOUT>
 __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict__ records, uint64_t rec_bytes,
-                                                       const uint64_t *__restrict__ offsets, int shuffle,
-                                                       float *__restrict__ dst, FieldGeom g,
+                                                       const uint64_t *__restrict__ offsets, int nblocks,
+                                                       int shuffle, float *__restrict__ dst, FieldGeom g,
                                                        uint32_t *__restrict__ err)
 {
     __shared__ float A[LDS_WORDS];
-    __shared__ uint32_t M[NT];
+    __shared__ uint32_t M[1024];
     __shared__ uint32_t wsum[NT / 64];
+    __shared__ uint32_t coarse_ok;
     const int t = threadIdx.x;
-    const int b = blockIdx.x;
-    const uint64_t ob = offsets[b];
-    if ((ob & 3u) || ob + 4 + MASK_BYTES > rec_bytes) {
-        if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_NBYTES);
-        return;
-    }
-    const uint8_t *rec = records + ob;
-    const uint32_t nb = *reinterpret_cast<const uint32_t *>(rec);
-    if (nb < MASK_BYTES || ((nb - MASK_BYTES) & 3u) || nb > SLOT32 || ob + 4 + nb > rec_bytes) {
-        if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_NBYTES);
-        return;
-    }
-    const uint32_t S = (nb - MASK_BYTES) >> 2;
-    const uint32_t *pay = reinterpret_cast<const uint32_t *>(rec + 4);
-    const uint32_t mw = pay[t];
-    M[t] = mw;
-    uint32_t *stage = reinterpret_cast<uint32_t *>(A);
-    const uint8_t *stage8 = reinterpret_cast<const uint8_t *>(A);
-    for (uint32_t i = t; i < S; i += NT) stage[i] = pay[MASK_BYTES / 4 + i];
-    __syncthreads();
+    const int G = gridDim.x;
+    int b = blockIdx.x;
+    uint64_t ob = b < nblocks ? offsets[b] : 0;
+    uint64_t ob_next = b + G < nblocks ? offsets[b + G] : 0;
+    RecPrefetch pf;
+    if (b < nblocks) fetch_record(records, rec_bytes, ob, t, pf);
+    for (; b < nblocks; b += G) {
+        const RecPrefetch cur = pf;
+        const uint64_t ob_cur = ob;
+        ob = ob_next;
+        if (b + 2 * G < nblocks) ob_next = offsets[b + 2 * G];
+        if (b + G < nblocks) fetch_record(records, rec_bytes, ob, t, pf);
 
-    uint32_t keep = 0;
-    for_each_run(t, [&](int q, int h, int, int, int word, int shift) {
-        const uint32_t hm = (h == 16) ? 0xffffu : ((1u << h) - 1u);
-        const uint32_t bits = (t < 2) ? hm : ((M[word] >> shift) & hm);
-        keep |= bits << (q * h);
-    });
-    // one scan carries both counts: survivors in the low 16 bits, mask
-    // popcount in the high 16 bits (each total is at most 32768 < 2^16)
-    uint32_t tot;
-    const uint32_t packed = block_excl_scan((uint32_t)__popc(keep) | ((uint32_t)__popc(mw) << 16), wsum, &tot);
-    const uint32_t first = packed & 0xffffu;
-    if ((tot & 0xffffu) != S || (tot >> 16) != S) {
-        if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_POPCOUNT);
-        return;
-    }
-    float v[32];
-    uint32_t idx = first;
-#pragma unroll
-    for (int j = 0; j < 32; ++j) {
-        float c = 0.0f;
-        if ((keep >> j) & 1u) {
-            uint32_t w;
-            if (shuffle)
-                w = (uint32_t)stage8[idx] | ((uint32_t)stage8[S + idx] << 8) | ((uint32_t)stage8[2 * S + idx] << 16) |
-                    ((uint32_t)stage8[3 * S + idx] << 24);
-            else
-                w = stage[idx];
-            c = __uint_as_float(w);
-            ++idx;
+        const uint32_t nb = cur.nb;
+        if (nb < MASK_BYTES || ((nb - MASK_BYTES) & 3u) || nb > SLOT32 || ob_cur + 4 + nb > rec_bytes) {
+            if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_NBYTES);
+            continue;
         }
-        v[j] = c;
-    }
-    __syncthreads();  // all survivors gathered: A becomes the coefficient array
-    for_each_run(t, [&](int q, int h, int addr, int stride, int, int) {
+        const uint32_t S = (nb - MASK_BYTES) >> 2;
+        const uint32_t *pay = reinterpret_cast<const uint32_t *>(records + ob_cur + 4);
+        __syncthreads();  // the previous block's last reads of A, M and coarse_ok are done
+        if (t == 0) coarse_ok = 1u;
 #pragma unroll
-        for (int i = 0; i < 16; ++i)
-            if (i < h) A[addr + i * stride] = v[q * h + i];
-    });
-    __syncthreads();
-    iwt_most_f32<WT>(A, t);
-    float r[32];
-    if constexpr (LAYOUT == LAYOUT_BLOCKS) {
-        const int z = t >> 5, y = t & 31;
-        lds_ld<32>(A, z * SP + y * RP, 1, r);
-        row<32, false, WT>(r);
-        float4 *p = reinterpret_cast<float4 *>(dst + (size_t)b * NB3 + (size_t)(z * 32 + y) * 32);
+        for (int k = 0; k < NROW; ++k) M[t + NT * k] = cur.mw[k];
+        uint32_t *stage = reinterpret_cast<uint32_t *>(A);
+        const uint8_t *stage8 = reinterpret_cast<const uint8_t *>(A);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) p[q] = make_float4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
-    } else {
-        const int y = t >> 5, z = t & 31;
-        lds_ld<32>(A, z * SP + y * RP, 1, r);
-        row<32, false, WT>(r);
-        size_t xs;
-        float *p = dst + field_row(g, b, y, z, &xs);
+        for (int k = 0; k < PF; ++k) {
+            const uint32_t i = (uint32_t)(k * NT + t);
+            if (i < S) stage[i] = cur.sv[k];
+        }
+        for (uint32_t i = PF * NT + t; i < S; i += NT) stage[i] = pay[MASK_BYTES / 4 + i];
+        __syncthreads();
+        // the 4^3 base is always stored (FullWaveletTransform.h:163-176): its 64
+        // mask bits are the low nibbles of words a1 + 32*a0, a0, a1 < 4
+        if (t < 16 && (M[(t & 3) + 32 * (t >> 2)] & 0xfu) != 0xfu) coarse_ok = 0u;
+
+        uint32_t keep[NROW];
 #pragma unroll
-        for (int x = 0; x < 32; ++x) p[x * xs] = r[x];
+        for (int k = 0; k < NROW; ++k) {
+            const int sg = t + NT * k;
+            keep[k] = 0;
+            for_each_run(sg, [&](int q, int h, int, int, int word, int shift) {
+                const uint32_t hm = (h == 16) ? 0xffffu : ((1u << h) - 1u);
+                const uint32_t bits = (sg < 2) ? hm : ((M[word] >> shift) & hm);
+                keep[k] |= bits << (q * h);
+            });
+        }
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan((uint32_t)__popc(keep[0]) | ((uint32_t)__popc(keep[1]) << 16), wsum, &tot);
+        // with all base bits set, the kept count equals the mask popcount
+        // that WaveletCompressor.cpp:235-251 checks against the payload size
+        if ((tot & 0xffffu) + (tot >> 16) != S || !coarse_ok) {
+            if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_POPCOUNT);
+            continue;
+        }
+        const uint32_t first[NROW] = {ex & 0xffffu, (tot & 0xffffu) + (ex >> 16)};
+        float v[NROW][32];
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) {
+            uint32_t idx = first[k];
+#pragma unroll
+            for (int j = 0; j < 32; ++j) {
+                float c = 0.0f;
+                if ((keep[k] >> j) & 1u) {
+                    uint32_t w;
+                    if (shuffle)
+                        w = (uint32_t)stage8[idx] | ((uint32_t)stage8[S + idx] << 8) |
+                            ((uint32_t)stage8[2 * S + idx] << 16) | ((uint32_t)stage8[3 * S + idx] << 24);
+                    else
+                        w = stage[idx];
+                    c = __uint_as_float(w);
+                    ++idx;
+                }
+                v[k][j] = c;
+            }
+        }
+        __syncthreads();  // all survivors gathered: A becomes the coefficient array
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) {
+            for_each_run(t + NT * k, [&](int q, int h, int addr, int stride, int, int) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i)
+                    if (i < h) A[addr + i * stride] = v[k][q * h + i];
+            });
+        }
+        __syncthreads();
+        iwt_most_f32<WT>(A, t);
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) {
+            float r[32];
+            lds_ld<32>(A, xrow_lds_base<LAYOUT>(t + NT * k), 1, r);
+            row<32, false, WT>(r);  // level 32, x-pass
+            store_xrow_f32<LAYOUT>(dst, g, b, t + NT * k, r);
+        }
     }
 }
 
@@ -508,6 +643,23 @@ static int wt_index(int wtype)
     }
 }
 
+// Workgroups that fit on the current device at once for a kernel with the
+// given block size (the persistent kernels use one 139 KB LDS block per CU).
+static int resident_groups(const void *kernel, int threads)
+{
+    static int cus[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev] = n;
+    }
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per <= 0) per = 1;
+    return cus[dev] * per;
+}
+
 static bool zb_ok(int zb) { return zb == 0 || zb == 4 || zb == 8 || zb == 12 || zb == 16; }
 
 template <int LAYOUT>
@@ -517,10 +669,12 @@ static int compress_slots_f32(const float *src, int nblocks, FieldGeom g, float 
     const int wt = wt_index(wtype);
     if (wt < 0 || !zb_ok(zerobits) || nblocks <= 0 || !src || !slots || !nb) return CZW_EINVAL;
     const uint32_t zkeep = zerobits ? ~((1u << zerobits) - 1u) : 0xffffffffu;
-    const dim3 grid(nblocks), blk(NT);
-    if (wt == 1) hipLaunchKernelGGL((k_compress_f32<1, LAYOUT>), grid, blk, 0, s, src, g, eps, zkeep, shuffle, slots, nb);
-    if (wt == 2) hipLaunchKernelGGL((k_compress_f32<2, LAYOUT>), grid, blk, 0, s, src, g, eps, zkeep, shuffle, slots, nb);
-    if (wt == 3) hipLaunchKernelGGL((k_compress_f32<3, LAYOUT>), grid, blk, 0, s, src, g, eps, zkeep, shuffle, slots, nb);
+    const void *fn = wt == 1 ? (const void *)k_compress_f32<1, LAYOUT>
+                   : wt == 2 ? (const void *)k_compress_f32<2, LAYOUT> : (const void *)k_compress_f32<3, LAYOUT>;
+    const dim3 grid(std::min(nblocks, resident_groups(fn, NT))), blk(NT);
+    if (wt == 1) hipLaunchKernelGGL((k_compress_f32<1, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, shuffle, slots, nb);
+    if (wt == 2) hipLaunchKernelGGL((k_compress_f32<2, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, shuffle, slots, nb);
+    if (wt == 3) hipLaunchKernelGGL((k_compress_f32<3, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, shuffle, slots, nb);
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
 
@@ -528,7 +682,7 @@ static int pack_records(const uint8_t *slots, size_t slot_bytes, const uint32_t 
                         uint64_t *offsets, hipStream_t s)
 {
     if (nblocks <= 0 || !slots || !nb || !records || !offsets || (slot_bytes & 3)) return CZW_EINVAL;
-    hipLaunchKernelGGL(k_scan_records, dim3(1), dim3(NT), 0, s, nb, nblocks, offsets);
+    hipLaunchKernelGGL(k_scan_records, dim3(1), dim3(1024), 0, s, nb, nblocks, offsets);
     hipLaunchKernelGGL(k_pack_records, dim3(nblocks), dim3(256), 0, s, slots, slot_bytes, nb, offsets, records);
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
@@ -552,10 +706,12 @@ static int decompress_f32(const void *records, size_t rec_bytes, const uint64_t 
     const int wt = wt_index(wtype);
     if (wt < 0 || nblocks <= 0 || !records || !offsets || !dst || !err) return CZW_EINVAL;
     const uint8_t *rec = static_cast<const uint8_t *>(records);
-    const dim3 grid(nblocks), blk(NT);
-    if (wt == 1) hipLaunchKernelGGL((k_decompress_f32<1, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, shuffle, dst, g, err);
-    if (wt == 2) hipLaunchKernelGGL((k_decompress_f32<2, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, shuffle, dst, g, err);
-    if (wt == 3) hipLaunchKernelGGL((k_decompress_f32<3, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, shuffle, dst, g, err);
+    const void *fn = wt == 1 ? (const void *)k_decompress_f32<1, LAYOUT>
+                   : wt == 2 ? (const void *)k_decompress_f32<2, LAYOUT> : (const void *)k_decompress_f32<3, LAYOUT>;
+    const dim3 grid(std::min(nblocks, resident_groups(fn, NT))), blk(NT);
+    if (wt == 1) hipLaunchKernelGGL((k_decompress_f32<1, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, nblocks, shuffle, dst, g, err);
+    if (wt == 2) hipLaunchKernelGGL((k_decompress_f32<2, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, nblocks, shuffle, dst, g, err);
+    if (wt == 3) hipLaunchKernelGGL((k_decompress_f32<3, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, nblocks, shuffle, dst, g, err);
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
 
