@@ -39,8 +39,61 @@ def build_hip():
     return out
 
 
+HDF5_PREFIX = os.environ.get("CZ_HDF5_PREFIX", "/opt/conda")
+BINDIR = os.path.join(HERE, "bin")
+TOOLS = {"hdf2cz": 1, "cz2hdf": 2, "cz2diff": 3}
+
+
+def build_host():
+    """libczhost.so: .cz container writer/reader (host C++, zlib, OpenMP)."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libczhost.so")
+    _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-Wall",
+          os.path.join(CSRC, "cz_file.cpp"), "-o", out + ".tmp", "-lz"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def have_hdf5():
+    return os.path.exists(os.path.join(HDF5_PREFIX, "include", "hdf5.h")) and \
+        os.path.exists(os.path.join(HDF5_PREFIX, "lib", "libhdf5.so"))
+
+
+def build_tools():
+    """hdf2cz / cz2hdf / cz2diff executables in cubismz_amd/bin."""
+    os.makedirs(BINDIR, exist_ok=True)
+    extra, libs = [], []
+    if have_hdf5():
+        # link libhdf5 through a private directory holding only a link to it,
+        # so that the HDF5 prefix's own (older) libstdc++ is never picked up
+        real = os.path.realpath(os.path.join(HDF5_PREFIX, "lib", "libhdf5.so"))
+        hdir = os.path.join(LIBDIR, "hdf5")
+        os.makedirs(hdir, exist_ok=True)
+        soname = os.path.basename(real).split(".so")[0] + ".so." + os.path.basename(real).split(".so.")[1].split(".")[0]
+        for name in ("libhdf5.so", soname):
+            link = os.path.join(hdir, name)
+            if os.path.lexists(link):
+                os.remove(link)
+            os.symlink(real, link)
+        extra = ["-DCZ_HAVE_HDF5", "-isystem", os.path.join(HDF5_PREFIX, "include")]
+        libs = ["-L" + hdir, "-lhdf5", "-Wl,-rpath,$ORIGIN/../lib/hdf5"]
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    outs = []
+    for name, k in TOOLS.items():
+        out = os.path.join(BINDIR, name)
+        # host-only C++ against the HIP runtime; all device code is in libczw.so
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", f"-DCZ_TOOL={k}", *extra,
+              "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(rocm, "include"),
+              os.path.join(CSRC, "cz_tools.cpp"), "-o", out + ".tmp",
+              "-L" + LIBDIR, "-lczw", "-lczhost", "-lz", *libs,
+              "-L" + os.path.join(rocm, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN/../lib"])
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
+
+
 def build_all():
-    return [build_hip()]
+    return [build_hip(), build_host(), *build_tools()]
 
 
 if __name__ == "__main__":

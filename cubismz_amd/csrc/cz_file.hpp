@@ -1,0 +1,104 @@
+// cz_file.hpp -- host side of the .cz container: chunked zlib stage, ASCII
+// header, block metadata and chunk look-up table.
+//
+// File layout (one writer, SerializerIO_WaveletCompression_MPI_Simple.h:347-466,
+// 489-591):
+//   @0    uint64 header displacement
+//   @8    "\n==============START-BINARY-OCEAN==============\n"      (48 bytes)
+//   @56   per subdomain: its zlib chunks, then uint64 lut[nchunks] holding each
+//         chunk's start relative to the subdomain's first byte
+//   @hd   ASCII header (SerializerIO_..._MPI_Simple.h:515-553)
+//         BlockMetadata[NBLOCKS]                                 (20 bytes each)
+//         "\n==============START-BINARY-LUT==============\n"       (46 bytes)
+//         HeaderLUT[SUBDOMAINS]                                  (12 bytes each)
+// A chunk is zlib (RFC 1950, level 6) of consecutive block records
+// [int32 nbytes][payload]; one writer thread puts ENTRIES consecutive blocks
+// in a chunk, ENTRIES = 4 MiB / (sizeof(WaveletCompressor) + 4) = 15 (float)
+// or 7 (double) (SerializerIO_..._MPI_Simple.h:68-76, 334-339).
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace cz {
+
+#pragma pack(push, 1)
+// WaveletSerializationTypes.h:16-18 (packed on-disk records)
+struct BlockMetadata {
+    int32_t idcompression, subid, ix, iy, iz;
+};
+struct HeaderLUT {
+    uint64_t aggregate_bytes;
+    int32_t nchunks;
+};
+struct CompressedBlock {
+    uint64_t start, extent;
+    int32_t subid;
+};
+#pragma pack(pop)
+static_assert(sizeof(BlockMetadata) == 20, "BlockMetadata must be 20 bytes");
+static_assert(sizeof(HeaderLUT) == 12, "HeaderLUT must be 12 bytes");
+static_assert(sizeof(CompressedBlock) == 20, "CompressedBlock must be 20 bytes");
+
+extern const char *const OCEAN_TITLE;  // 48 bytes
+extern const char *const LUT_TITLE;    // 46 bytes
+
+enum Encoder { ENC_NONE = 0, ENC_ZLIB = 1 };
+
+// Name recorded for a wavelet type (FullWaveletTransform.h:28-35); wtype < 0
+// means no wavelet stage (raw blocks, the reference's "default" build).
+const char *wavelet_name(int wtype);
+int wavelet_type(const std::string &name);  // inverse of wavelet_name, -1 for "none"
+
+// Blocks per chunk of one writer thread for sizeof(Real) = prec.
+int entries_per_chunk(int prec);
+
+struct Params {
+    int prec = 4;                  // sizeof(Real)
+    int bpd[3] = {1, 1, 1};        // blocks per dimension (global)
+    int sub_bpd[3] = {1, 1, 1};    // blocks per dimension of one subdomain
+    int wtype = 3;                 // < 0: raw blocks ("Wavelets: none")
+    double threshold = 0.0;        // printed as Real
+    int encoder = ENC_ZLIB;
+    bool halffloat = false;
+};
+
+// The ASCII header text exactly as the reference writes it.
+std::string make_header(const Params &p);
+
+// Deflate one chunk the way the reference writer does (deflateInit with
+// Z_DEFAULT_COMPRESSION, Z_FINISH).  Returns false on a zlib error.
+bool deflate_chunk(const uint8_t *in, size_t n, std::vector<uint8_t> &out);
+// Inflate a whole chunk; returns false unless inflate ends with Z_STREAM_END.
+bool inflate_chunk(const uint8_t *in, size_t n, std::vector<uint8_t> &out);
+
+// Serialize one subdomain's record stream (block order, x fastest) into a
+// complete single-subdomain .cz image.  records/offsets as in czw.h.
+int build_cz(const Params &p, const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads,
+             std::vector<uint8_t> &file);
+
+struct FileInfo {
+    Params p;
+    int nblocks = 0;
+    uint64_t header_displacement = 0;
+    std::vector<CompressedBlock> idx2chunk;  // per block id ix + bx*(iy + by*iz)
+    std::vector<uint64_t> chunk_start;       // absolute file offsets
+    std::vector<uint64_t> chunk_size;
+};
+
+// Parse header, metadata and look-up tables (Reader_WaveletCompression.h:255-560).
+// Returns 0 or a negative error; *msg receives a description.
+int parse_cz(const std::vector<uint8_t> &file, FileInfo &info, std::string *msg);
+
+// Inflate every chunk once and gather all block records in block order:
+// records receives the concatenated inflated chunks and offsets[b] the
+// position of block b's int32 size word (offsets has nblocks+1 entries, the
+// last = records.size()).
+int gather_records(const std::vector<uint8_t> &file, const FileInfo &info, int nthreads, std::vector<uint8_t> &records,
+                   std::vector<uint64_t> &offsets, std::string *msg);
+
+bool read_file(const std::string &path, std::vector<uint8_t> &data);
+bool write_file(const std::string &path, const std::vector<uint8_t> &data);
+
+}  // namespace cz

@@ -1,0 +1,145 @@
+"""Python front end of the host .cz container library (lib/libczhost.so).
+
+Mirrors the reference's serializer/reader pair around the block compressor:
+``write`` = SerializerIO_WaveletCompression_MPI_SimpleBlocking::_write for one
+writer (Compressor/source/SerializerIO_WaveletCompression_MPI_Simple.h:489-644),
+``CzFile`` = Reader_WaveletCompression::load_file + the chunk inflate of
+load_block2 (Compressor/source/Reader_WaveletCompression.h:255-560, 635-698),
+done once per chunk for the whole file.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "lib", "libczhost.so")
+ENCODERS = {"none": 0, "zlib": 1}
+
+_lib = None
+
+
+class CzFileError(RuntimeError):
+    pass
+
+
+def load_library():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise CzFileError(f"{LIBPATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIBPATH)
+        vp, i = C.c_void_p, C.c_int
+        L.czh_entries_per_chunk.argtypes = [i]
+        L.czh_write.argtypes = [C.c_char_p, i, vp, i, C.c_double, i, vp, vp, i, i]
+        L.czh_open.restype = vp
+        L.czh_open.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t]
+        L.czh_close.argtypes = [vp]
+        L.czh_info.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.czh_file_bytes.restype = C.c_uint64
+        L.czh_file_bytes.argtypes = [vp]
+        L.czh_gather.argtypes = [vp, i, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.POINTER(C.c_void_p)]
+        L.czh_free.argtypes = [vp]
+        L.czh_deflate.argtypes = [vp, C.c_uint64, vp, C.c_uint64, C.POINTER(C.c_uint64)]
+        _lib = L
+    return _lib
+
+
+def entries_per_chunk(prec):
+    return load_library().czh_entries_per_chunk(int(prec))
+
+
+def write(path, records, offsets, bpd, prec, wtype, threshold, encoder="zlib", nthreads=0):
+    """Write a single-subdomain .cz from a host record stream in block order."""
+    rec = np.ascontiguousarray(records, dtype=np.uint8)
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    b = np.ascontiguousarray(bpd, dtype=np.int32)
+    nblocks = int(b.prod())
+    if off.size != nblocks + 1:
+        raise CzFileError("offsets must have nblocks+1 entries")
+    rc = load_library().czh_write(os.fsencode(path), int(prec), b.ctypes.data, int(wtype), float(threshold),
+                                  ENCODERS[encoder], rec.ctypes.data, off.ctypes.data, nblocks, int(nthreads))
+    if rc:
+        raise CzFileError(f"czh_write failed ({rc})")
+
+
+def deflate(data):
+    """The writer's per-chunk zlib stage."""
+    d = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8))
+    out = np.empty(d.size + d.size // 100 + 1024, np.uint8)
+    n = C.c_uint64()
+    if load_library().czh_deflate(d.ctypes.data, d.size, out.ctypes.data, out.size, C.byref(n)):
+        raise CzFileError("deflate failed")
+    return out[:n.value].tobytes()
+
+
+class CzFile:
+    """A parsed .cz file."""
+
+    def __init__(self, path):
+        L = load_library()
+        err = C.create_string_buffer(256)
+        h = L.czh_open(os.fsencode(path), err, 256)
+        if not h:
+            raise CzFileError(f"{path}: {err.value.decode()}")
+        self._h = h
+        prec, wt, enc = C.c_int(), C.c_int(), C.c_int()
+        thr = C.c_double()
+        bpd = (C.c_int * 3)()
+        sub = (C.c_int * 3)()
+        L.czh_info(h, C.byref(prec), bpd, sub, C.byref(wt), C.byref(thr), C.byref(enc))
+        self.prec, self.wtype, self.threshold = prec.value, wt.value, thr.value
+        self.encoder = {v: k for k, v in ENCODERS.items()}[enc.value]
+        self.bpd, self.sub_bpd = tuple(bpd), tuple(sub)
+        self.nblocks = int(np.prod(self.bpd))
+        self.file_bytes = int(L.czh_file_bytes(h))
+
+    @property
+    def shape(self):
+        return tuple(32 * b for b in self.bpd)
+
+    def records(self, nthreads=0):
+        """(records, offsets): all block records in block order, each chunk inflated once."""
+        L = load_library()
+        rp, op = C.c_void_p(), C.c_void_p()
+        nb = C.c_uint64()
+        rc = L.czh_gather(self._h, int(nthreads), C.byref(rp), C.byref(nb), C.byref(op))
+        if rc:
+            raise CzFileError(f"czh_gather failed ({rc})")
+        try:
+            rec = np.ctypeslib.as_array(C.cast(rp, C.POINTER(C.c_uint8)), shape=(max(nb.value, 1),))[:nb.value].copy()
+            off = np.ctypeslib.as_array(C.cast(op, C.POINTER(C.c_uint64)), shape=(self.nblocks + 1,)).copy()
+        finally:
+            L.czh_free(rp)
+            L.czh_free(op)
+        return rec, off
+
+    def close(self):
+        if self._h:
+            load_library().czh_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def records_from_slots(slots, nbytes):
+    """Frame per-block payload slots as a record stream [int32 nbytes][payload]."""
+    nb = np.asarray(nbytes, dtype=np.int64)
+    off = np.zeros(nb.size + 1, np.uint64)
+    off[1:] = np.cumsum(nb + 4)
+    rec = np.empty(int(off[-1]), np.uint8)
+    for b in range(nb.size):
+        o = int(off[b])
+        rec[o:o + 4] = np.frombuffer(np.int32(nb[b]).tobytes(), np.uint8)
+        rec[o + 4:o + 4 + nb[b]] = slots[b, :nb[b]]
+    return rec, off

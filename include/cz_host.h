@@ -1,0 +1,60 @@
+/*
+ * cz_host.h -- C ABI of the host-side .cz container library (libczhost.so).
+ *
+ * Replaces the reference's serializer and reader around the block compressor:
+ *  - czh_write: the chunking/zlib/LUT/header/metadata part of
+ *    SerializerIO_WaveletCompression_MPI_SimpleBlocking::_write
+ *    (Compressor/source/SerializerIO_WaveletCompression_MPI_Simple.h:101-184,
+ *    489-591) and _to_file (:347-466) for one writer;
+ *  - czh_open/czh_info: Reader_WaveletCompression::load_file
+ *    (Compressor/source/Reader_WaveletCompression.h:255-560);
+ *  - czh_gather: the inflate + record lookup of load_block2 (:635-698), done
+ *    once per chunk for the whole file instead of once per block.
+ * All buffers here are HOST memory.  The device side is include/czw.h.
+ */
+#ifndef CZ_HOST_H
+#define CZ_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { CZH_OK = 0, CZH_EINVAL = -1, CZH_EIO = -2, CZH_EZLIB = -3, CZH_ECORRUPT = -4 };
+enum { CZH_ENCODER_NONE = 0, CZH_ENCODER_ZLIB = 1 };
+
+typedef struct czh_file czh_file;
+
+/* Blocks per chunk of one writer thread: 15 for prec 4, 7 for prec 8. */
+int czh_entries_per_chunk(int prec);
+
+/* Write a single-subdomain .cz file from a host record stream in block order
+ * (block b = ix + bpd[0]*(iy + bpd[1]*iz)).  wtype < 0 records "Wavelets: none"
+ * (raw blocks).  threshold is printed as the Real of the given precision.
+ * nthreads <= 0 keeps the OpenMP default for the per-chunk deflate. */
+int czh_write(const char *path, int prec, const int bpd[3], int wtype, double threshold, int encoder,
+              const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads);
+
+/* Read and parse a .cz file (any number of subdomains).  Returns NULL on
+ * error with a message in errbuf. */
+czh_file *czh_open(const char *path, char *errbuf, size_t errlen);
+void czh_close(czh_file *f);
+int czh_info(const czh_file *f, int *prec, int bpd[3], int sub_bpd[3], int *wtype, double *threshold, int *encoder);
+uint64_t czh_file_bytes(const czh_file *f);
+
+/* Inflate every chunk once; *records (malloc'ed, free with czh_free) holds
+ * the records of all blocks and (*offsets)[b] the position of block b's int32
+ * size word; *offsets has nblocks+1 entries. */
+int czh_gather(czh_file *f, int nthreads, uint8_t **records, uint64_t *records_bytes, uint64_t **offsets);
+void czh_free(void *p);
+
+/* The writer's per-chunk deflate (zlib level Z_DEFAULT_COMPRESSION, Z_FINISH). */
+int czh_deflate(const uint8_t *in, uint64_t n, uint8_t *out, uint64_t cap, uint64_t *produced);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CZ_HOST_H */

@@ -59,6 +59,9 @@ def lib():
             getattr(L, f"czo_decompress_field_{s}").argtypes = [
                 C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                 C.c_void_p, C.c_size_t, C.c_void_p]
+        for s in ("f32", "f64"):
+            getattr(L, f"czo_diff_metrics_{s}").argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                                             C.c_void_p]
         L.czo_set_threads.argtypes = [C.c_int]
         L.czo_get_threads.restype = C.c_int
         _lib = L
@@ -146,6 +149,29 @@ def decompress_field(slots, nbytes, shape, wtype, prec, shuffle=False, b0=0, b1=
     if rc:
         raise ValueError("corrupt payload")
     return f
+
+
+def res_line(decoded, reference, file_bytes):
+    """The two RES lines cz2diff prints (Tools/cz2diff.cpp:221-261), from the
+    oracle's restatement of its accumulation order."""
+    import math
+
+    dt, s = _dt(decoded.dtype.itemsize)
+    a = np.ascontiguousarray(decoded, dtype=dt)
+    b = np.ascontiguousarray(reference, dtype=dt)
+    out = np.zeros(9)
+    getattr(lib(), f"czo_diff_metrics_{s}")(_ptr(a), _ptr(b), *a.shape, _ptr(out))
+    e_inf, e_1, e_2, n_inf, n_1, n_2, mn, mx, n = out
+    n = int(n)
+    unc = a.itemsize * n
+    mse = e_2 / n
+    psnr = 20 * math.log10((mx - mn) / (2 * math.sqrt(mse))) if mse > 0 else float("inf")
+    head = "RES: %12s %12s %12s %12s %12s %12s %12s %12s" % ("CR", "rel(e_inf)", "rel(e_1)", "mean(e_1)", "rel(e_2)",
+                                                            "mean(e_2)", "BPS", "PSNR")
+    vals = "RES: %12.2f %.6e %.6e %.6e %.6e %.6e %12.04f %12.04f" % (
+        unc / file_bytes, e_inf / n_inf, e_1 / n_1, e_1 / n, math.sqrt(e_2) / math.sqrt(n_2), math.sqrt(e_2) / n,
+        file_bytes * 8.0 / n, psnr)
+    return head, vals
 
 
 # ---------------------------------------------------------------------------

@@ -1,0 +1,195 @@
+"""The host .cz container (libczhost.so) on CPU: layout, chunking, zlib stage,
+reader, and the reference's published end-to-end known answers.
+
+Known answers (Tests/Test1/cselab_ref_run_all.txt:12 and
+Tests/Test1_dp/cselab_ref_run_all.txt:12): demo.h5 / demo_dp.h5, wavelet
+type 3, threshold 5e-5, zlib, one writer thread, compared by cz2diff against
+the uncompressed data.  Here the per-block payloads come from the oracle; the
+GPU versions of these tests live in test_gpu_tools.py.
+"""
+import ctypes as C
+import lzma
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from cubismz_amd import czfile
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+PUBLISHED = {
+    "f32": "RES:       256.23 8.722263e-04 8.271610e-05 3.972721e-05 1.046102e-04 3.611734e-08       0.1249      78.2226",
+    "f64": "RES:       348.36 8.717107e-04 8.271310e-05 3.972577e-05 1.046065e-04 3.611606e-08       0.1837      78.2229",
+}
+
+
+def demo(prec):
+    dt = "<f4" if prec == "f32" else "<f8"
+    raw = lzma.decompress(open(os.path.join(GOLD, f"demo_{prec}.raw.xz"), "rb").read())
+    return np.frombuffer(raw, dt).reshape(128, 128, 128)
+
+
+@pytest.fixture(scope="module")
+def host():
+    return czfile.load_library()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_published_known_answer(oracle, host, tmp_path, prec):
+    d = demo(prec)
+    s = 4 if prec == "f32" else 8
+    slots, nb = oracle.compress_field(d, 5e-5, 3)
+    rec, off = czfile.records_from_slots(slots, nb)
+    path = tmp_path / "tmp.cz"
+    czfile.write(str(path), rec, off, (4, 4, 4), s, 3, 5e-5)
+    size = os.path.getsize(path)
+    dec = oracle.decompress_field(slots, nb, d.shape, 3, prec)
+    head, vals = oracle.res_line(dec, d, size)
+    assert vals == PUBLISHED[prec], (vals, size)
+
+
+def test_layout_and_header_text(oracle, host, tmp_path):
+    """Byte layout of SerializerIO_WaveletCompression_MPI_Simple.h:347-466,497-591."""
+    d = demo("f32")
+    slots, nb = oracle.compress_field(d, 5e-5, 3)
+    rec, off = czfile.records_from_slots(slots, nb)
+    path = tmp_path / "a.cz"
+    czfile.write(str(path), rec, off, (4, 4, 4), 4, 3, 5e-5)
+    b = path.read_bytes()
+    hd = struct.unpack_from("<Q", b, 0)[0]
+    assert b[8:56] == b"\n==============START-BINARY-OCEAN==============\n"
+    header = (b"\n==============START-ASCI-HEADER==============\nEndianess: little\nsizeofReal: 4\n"
+              b"sizeofsize_t: 8\nsizeofBlockMetadata: 20\nsizeofHeaderLUT: 12\nsizeofCompressedBlock: 20\n"
+              b"Blocksize: 32\nBlocks: 4 x 4 x 4\nExtent: 1 1 1\nSubdomainBlocks: 4 x 4 x 4\nHalfFloat: no\n"
+              b"Wavelets: AverageInterpWavelet3rdOrder\nWaveletThreshold: 5e-05\nEncoder: zlib\n"
+              b"==============START-BINARY-METABLOCKS==============\n")
+    assert b[hd:hd + len(header)] == header
+    m0 = hd + len(header)
+    meta = np.frombuffer(b[m0:m0 + 64 * 20], np.int32).reshape(64, 5)
+    i = np.arange(64)
+    assert np.array_equal(meta[:, 0], i // 15) and np.array_equal(meta[:, 1], i % 15)
+    assert np.array_equal(meta[:, 2], i % 4) and np.array_equal(meta[:, 3], (i // 4) % 4)
+    assert np.array_equal(meta[:, 4], i // 16)
+    t0 = m0 + 64 * 20
+    assert b[t0:t0 + 46] == b"\n==============START-BINARY-LUT==============\n"
+    agg, nch = struct.unpack_from("<Qi", b, t0 + 46)
+    assert len(b) == t0 + 46 + 12 and nch == 5 and 56 + agg == hd
+    lut = np.frombuffer(b[56 + agg - 8 * nch:56 + agg], np.uint64)
+    # each chunk is zlib level 6 of 15 consecutive records
+    for c in range(nch):
+        end = lut[c + 1] if c + 1 < nch else agg - 8 * nch
+        lo, hi = int(off[15 * c]), int(off[min(15 * c + 15, 64)])
+        assert b[56 + int(lut[c]):56 + int(end)] == czfile.deflate(rec[lo:hi].tobytes())
+
+
+def test_entries_per_chunk_matches_reference_sizeof(oracle, host):
+    """ENTRIES = 4 MiB / (sizeof(WaveletCompressor) + 4) (SerializerIO...:68-76)."""
+    assert czfile.entries_per_chunk(4) == 15 and czfile.entries_per_chunk(8) == 7
+    for prec, s in (("f32", 4), ("f64", 8)):
+        if oracle.ref_available(prec):
+            size = oracle.ref(prec).rf_sizeof_compressor()
+            assert czfile.entries_per_chunk(s) == (4 * 1024 * 1024) // (size + 4)
+
+
+def test_chunk_deflate_equals_reference_inplace(host):
+    """The writer's one-shot deflate gives the bytes of the reference's
+    in-place chunk deflate (CompressionEncoders.h:113-190), where built."""
+    path = os.path.join(os.path.dirname(GOLD), "..", "oracle", "_ref", "libczref_zlib.so")
+    if not os.path.exists(path):
+        pytest.skip("oracle/_ref not built here")
+    L = C.CDLL(path)
+    L.rf_deflate_inplace.restype = C.c_long
+    L.rf_deflate_inplace.argtypes = [C.c_void_p, C.c_uint, C.c_uint]
+    rng = np.random.default_rng(1)
+    for n in (1, 100, 4100 * 15, 2 * 1024 * 1024):
+        data = (rng.integers(0, 4, n) * rng.integers(0, 255, n)).astype(np.uint8)
+        buf = data.copy()
+        m = L.rf_deflate_inplace(buf.ctypes.data, n, n)
+        want = czfile.deflate(data.tobytes())
+        if m >= 0:
+            assert buf[:m].tobytes() == want
+
+
+def test_reader_roundtrip_and_raw_blocks(oracle, host, tmp_path):
+    rng = np.random.default_rng(2)
+    nb = 2 * 3 * 1
+    blocks = rng.standard_normal((nb, 32, 32, 32)).astype(np.float64)
+    # raw-block records, the reference's uncompressed build ("Wavelets: none")
+    sizes = np.full(nb, 32768 * 8)
+    rec, off = czfile.records_from_slots(blocks.reshape(nb, -1).view(np.uint8), sizes)
+    for enc in ("zlib", "none"):
+        path = tmp_path / f"raw_{enc}.cz"
+        czfile.write(str(path), rec, off, (2, 3, 1), 8, -1, 0.0, encoder=enc)
+        with czfile.CzFile(str(path)) as f:
+            assert (f.prec, f.bpd, f.wtype, f.encoder) == (8, (2, 3, 1), -1, enc)
+            r2, o2 = f.records()
+        assert np.array_equal(r2, rec) and np.array_equal(o2, off)
+
+
+def test_reader_rejects_corruption(oracle, host, tmp_path):
+    d = demo("f32")[:64, :64, :32]
+    slots, nb = oracle.compress_field(np.ascontiguousarray(d), 1e-3, 1)
+    rec, off = czfile.records_from_slots(slots, nb)
+    path = tmp_path / "c.cz"
+    czfile.write(str(path), rec, off, (2, 2, 1), 4, 1, 1e-3)
+    good = path.read_bytes()
+    hd = struct.unpack_from("<Q", good, 0)[0]
+    cases = {
+        "truncated": good[:hd + 20],
+        "bad_displacement": struct.pack("<Q", len(good) + 5) + good[8:],
+        "bad_real": good.replace(b"sizeofReal: 4", b"sizeofReal: 5"),
+        "bad_encoder": good.replace(b"Encoder: zlib", b"Encoder: lz77"),
+    }
+    for name, data in cases.items():
+        p = tmp_path / f"{name}.cz"
+        p.write_bytes(data)
+        with pytest.raises(czfile.CzFileError):
+            czfile.CzFile(str(p))
+    # a corrupted chunk must fail to inflate (the reference accepts it:
+    # CompressionEncoders.h:64 treats negative inflate codes as success)
+    bad = bytearray(good)
+    for k in range(60, 90):
+        bad[k] ^= 0x5A
+    p = tmp_path / "chunk.cz"
+    p.write_bytes(bytes(bad))
+    with czfile.CzFile(str(p)) as f, pytest.raises(czfile.CzFileError):
+        f.records()
+
+
+def test_multi_subdomain_reader(oracle, host, tmp_path):
+    """A file of two subdomains (two writer ranks, SerializerIO...:347-466 with
+    MPI_Exscan offsets) is read with per-subdomain look-up tables
+    (Reader_WaveletCompression.h:465-522)."""
+    rng = np.random.default_rng(3)
+    blocks = (rng.standard_normal((4, 32, 32, 32)) * 1e-2).astype(np.float32)
+    pays = [oracle.compress(b, 1e-3, 3) for b in blocks]
+    # two ranks, each with a 1x2x1 subdomain; global blocks 2x2x1
+    ranks = [[0, 2], [1, 3]]  # global block ids (x fastest) owned by rank 0 / rank 1
+    oceans, luts, metas, hls = [], [], [], []
+    gchunk = 0
+    for blist in ranks:
+        recs = b"".join(struct.pack("<i", len(pays[g])) + pays[g].tobytes() for g in blist)
+        chunk = czfile.deflate(recs)
+        ocean = chunk + struct.pack("<Q", 0)
+        oceans.append(ocean)
+        hls.append(struct.pack("<Qi", len(ocean), 1))
+        for sub, g in enumerate(blist):
+            metas.append(struct.pack("<5i", 0, sub, g % 2, g // 2, 0))
+    body = b"".join(oceans)
+    header = (b"\n==============START-ASCI-HEADER==============\nEndianess: little\nsizeofReal: 4\n"
+              b"sizeofsize_t: 8\nsizeofBlockMetadata: 20\nsizeofHeaderLUT: 12\nsizeofCompressedBlock: 20\n"
+              b"Blocksize: 32\nBlocks: 2 x 2 x 1\nExtent: 1 1 0.5\nSubdomainBlocks: 1 x 2 x 1\nHalfFloat: no\n"
+              b"Wavelets: AverageInterpWavelet3rdOrder\nWaveletThreshold: 0.001\nEncoder: zlib\n"
+              b"==============START-BINARY-METABLOCKS==============\n")
+    data = (struct.pack("<Q", 56 + len(body)) + b"\n==============START-BINARY-OCEAN==============\n" + body +
+            header + b"".join(metas) + b"\n==============START-BINARY-LUT==============\n" + b"".join(hls))
+    p = tmp_path / "two.cz"
+    p.write_bytes(data)
+    with czfile.CzFile(str(p)) as f:
+        assert f.bpd == (2, 2, 1) and f.sub_bpd == (1, 2, 1)
+        rec, off = f.records()
+    for g in range(4):
+        o = int(off[g])
+        n = struct.unpack_from("<i", rec.tobytes(), o)[0]
+        assert rec[o + 4:o + 4 + n].tobytes() == pays[g].tobytes()
