@@ -3,8 +3,10 @@
 
 Workload (the metric's configuration): one 512^3 float32 cavitation-like
 pressure field (background 1.0 + 200 bubbles, cubismz_amd.fields.bubble_field,
-seed 1), 16^3 blocks of 32^3, eps = 1e-3, wavelet type 3 (the reference CLI
-default, Tools/Test_IO_Compressed.h:305), no zerobits, no shuffle.
+seed 1), 16^3 blocks of 32^3, eps = 1e-3, the fourth-order interpolating
+wavelet (wavelet type 1, WaveletsOnInterval.h:33-51, 97-153), no zerobits, no
+shuffle.  The reference CLI's default, the third-order average-interpolating
+wavelet (type 3, Tools/Test_IO_Compressed.h:305), is timed beside it.
 
 A step = compress (fwt + threshold + mask + survivor stream + record framing)
 + decompress (decode + iwt) of the whole field, input already resident in
@@ -38,7 +40,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--eps", type=float, default=1e-3)
-    ap.add_argument("--wtype", type=int, default=3)
+    ap.add_argument("--wtype", type=int, default=1)
+    ap.add_argument("--extra-wtypes", type=str, default="3", help="comma list of other wavelet types to time")
     ap.add_argument("--zerobits", type=int, default=0)
     ap.add_argument("--shuffle", type=int, default=0)
     ap.add_argument("--bubbles", type=int, default=200)
@@ -73,7 +76,7 @@ def main():
 
     slot = czw.slot_bytes(prec)
     slots = torch.empty(nblocks * slot, dtype=torch.uint8, device=dev)
-    nbytes = torch.empty(nblocks, dtype=torch.int32, device=dev)
+    nbytes = torch.empty(2 * nblocks, dtype=torch.int32, device=dev)
     records = torch.empty(czw.records_bound(nblocks, prec), dtype=torch.uint8, device=dev)
     offsets = torch.empty(nblocks + 1, dtype=torch.int64, device=dev)
     out = torch.empty_like(field)
@@ -82,10 +85,10 @@ def main():
     def step(ev=None):
         if ev:
             ev[0].record()
-        czw.compress_field_slots(field, a.eps, a.wtype, a.zerobits, a.shuffle, slots, nbytes)
+        czw.compress_field_slots(field, a.eps, a.wtype, a.zerobits, slots, nbytes)
         if ev:
             ev[1].record()
-        czw.pack_records(slots, slot, nbytes, nblocks, records, offsets)
+        czw.pack_records(slots, slot, nbytes, nblocks, a.shuffle, records, offsets)
         if ev:
             ev[2].record()
         czw.decompress_field(records, offsets, shape, torch.float32, a.wtype, a.shuffle, out=out, err=err,
@@ -115,8 +118,29 @@ def main():
     ms_per_step = wall / a.steps * 1e3
 
     field_bytes = field.numel() * prec
+    other = {}
+    for w in [int(x) for x in a.extra_wtypes.split(",") if x.strip()]:
+        if w == a.wtype:
+            continue
+        for _ in range(a.warmup):
+            czw.compress_field_slots(field, a.eps, w, a.zerobits, slots, nbytes)
+            czw.pack_records(slots, slot, nbytes, nblocks, a.shuffle, records, offsets)
+            czw.decompress_field(records, offsets, shape, torch.float32, w, a.shuffle, out=out, err=err, check=False)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(a.steps):
+            czw.compress_field_slots(field, a.eps, w, a.zerobits, slots, nbytes)
+            czw.pack_records(slots, slot, nbytes, nblocks, a.shuffle, records, offsets)
+            czw.decompress_field(records, offsets, shape, torch.float32, w, a.shuffle, out=out, err=err, check=False)
+        e1.record()
+        torch.cuda.synchronize()
+        tw = e0.elapsed_time(e1) / a.steps * 1e-3
+        other[f"wtype{w}"] = {"GBps": round(field_bytes / tw / 1e9, 3), "ms_per_step": round(tw * 1e3, 4)}
+    # restore the headline wavelet's records for the checks below
+    step()
+    torch.cuda.synchronize()
     off = offsets.cpu().numpy()
-    nb = nbytes.cpu().numpy().astype(np.int64)
+    nb = nbytes[:nblocks].cpu().numpy().astype(np.int64)
     payload = int(nb.sum())
     surv = int(((nb - 4096) // prec).sum())
     rho = surv / field.numel()
@@ -183,6 +207,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
                      "alg_bytes_per_launch": kern[dom][0], "avg_launch_ms": round(kern[dom][1] * 1e3, 4)},
         "step_roofline_frac": round((alg_c + alg_d) / t_step / HBM_PEAK, 4),
+        "other_wtypes": other,
         "compression": {"payload_bytes": payload, "survivor_fraction": round(rho, 5),
                         "ratio_vs_records": round(field_bytes / float(off[-1]), 3),
                         "linf": err_inf, "psnr_db": round(float(psnr), 4)},

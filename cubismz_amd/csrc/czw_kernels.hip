@@ -202,7 +202,7 @@ CZW_DI void fwt16_8_f32(float *A, int t)
 // caller fuses with the store: level 8 (x, z, y), level 16 (y fused, x, z),
 // level 32 (z fused, y).
 template <int WT>
-CZW_DI void iwt_most_f32(float *A, int t)
+CZW_DI void iwt16_8_f32(float *A, int t)
 {
     {
         float r[16];
@@ -236,6 +236,12 @@ CZW_DI void iwt_most_f32(float *A, int t)
         }
         __syncthreads();
     }
+}
+
+template <int WT>
+CZW_DI void iwt_most_f32(float *A, int t)
+{
+    iwt16_8_f32<WT>(A, t);
 #pragma unroll
     for (int k = 0; k < NROW; ++k) {  // level 16 z-pass, then level 32 z-pass
         const int q = t + NT * k, y = q >> 5, x = q & 31;
@@ -326,14 +332,69 @@ CZW_DI void store_xrow_f32(float *__restrict__ dst, const FieldGeom &g, int b, i
 // and loads block b+G into registers while block b is transformed, so the
 // HBM reads of one block overlap the arithmetic of the previous one.  Block
 // b's payload goes to slots + b*SLOT32 and its size to nbytes[b].
+// LDS address of bit j (0..31) of stream segment g (see for_each_run).
+CZW_DI int seg_lds_addr(int g, int j)
+{
+    if (g < 16) {
+        const int s = 32 * g + j, code = s >> 6, k = s & 63;
+        const int a0 = 4 * ((code >> 2) & 1) + (k >> 4), a1 = 4 * ((code >> 1) & 1) + ((k >> 2) & 3);
+        return a0 * SP + a1 * RP + 4 * (code & 1) + (k & 3);
+    } else if (g < 128) {
+        const int s = 32 * g + j, code = s >> 9, k = s & 511;
+        const int a0 = 8 * ((code >> 2) & 1) + (k >> 6), a1 = 8 * ((code >> 1) & 1) + ((k >> 3) & 7);
+        return a1 * SP + (8 * (code & 1) + (k & 7)) * RP + a0;
+    } else {
+        const int s = 32 * g + j, code = s >> 12, k = s & 4095;
+        const int a0 = 16 * ((code >> 2) & 1) + (k >> 8), a1 = 16 * ((code >> 1) & 1) + ((k >> 4) & 15);
+        return (16 * (code & 1) + (k & 15)) * SP + a0 * RP + a1;
+    }
+}
+
+// Slot layout written by the compressor (block b at slots + b*SLOT32):
+//   [mask 4096 B][survivors of the [0,16)^3 corner, stream order, <= 4096
+//   words][level-32 survivors, stream order], both before the byte shuffle.
+// nbytes[b] = payload bytes 4096 + 4*S, nbytes[nblocks + b] = corner count.
+constexpr int CORNER_MAX = 4096;
+constexpr int NRUN32 = 1792;  // level-32 stream runs: 7 codes x 16 x 16 runs of 16
+constexpr int STAGE32_AT = 16 * SP;                      // the z >= 16 half of A
+constexpr int STAGE32 = ((LDS_WORDS - STAGE32_AT - 64) / 4) * 4;  // staged survivors (64 dummy words follow)
+
+// Stream positions [8p, 8p+8) of the [0,16)^3 corner, p < 512 (see
+// for_each_run): two runs of 4 for p < 64 (base 4^3 and level-8 details),
+// one run of 8 otherwise (level-16 details).
+template <typename F>
+CZW_DI void corner_piece(int p, F &&f)
+{
+    if (p < 64) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int s = 8 * p + 4 * q;
+            const int code = s >> 6, k = s & 63;
+            const int a0 = 4 * ((code >> 2) & 1) + (k >> 4);
+            const int a1 = 4 * ((code >> 1) & 1) + ((k >> 2) & 3);
+            const int o2 = 4 * (code & 1);
+            f(q, 4, a0 * SP + a1 * RP + o2, 1, a1 + 32 * a0, o2);
+        }
+    } else {
+        const int s = 8 * p;
+        const int code = s >> 9, k = s & 511;
+        const int a0 = 8 * ((code >> 2) & 1) + (k >> 6);
+        const int a1 = 8 * ((code >> 1) & 1) + ((k >> 3) & 7);
+        const int o2 = 8 * (code & 1);
+        f(0, 8, a1 * SP + o2 * RP + a0, RP, a1 + 32 * a0, o2);
+    }
+}
+
 template <int WT, int LAYOUT>
 __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ src, FieldGeom g, int nblocks,
-                                                     float eps, uint32_t zkeep, int shuffle,
-                                                     uint8_t *__restrict__ slots, uint32_t *__restrict__ nbytes)
+                                                     float eps, uint32_t zkeep, uint8_t *__restrict__ slots,
+                                                     uint32_t *__restrict__ nbytes)
 {
-    __shared__ float A[LDS_WORDS];
+    __shared__ __align__(16) float A[LDS_WORDS];
     __shared__ uint32_t M[1024];
+    __shared__ uint32_t cnt[2048];  // level-32 run counts, then their exclusive offsets
     __shared__ uint32_t wsum[NT / 64];
+    __shared__ uint32_t s32_total;
     const int t = threadIdx.x;
     const int G = gridDim.x;
     float nx[NROW][32];
@@ -343,81 +404,160 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
         for (int k = 0; k < NROW; ++k) load_xrow_f32<LAYOUT>(src, g, b, t + NT * k, nx[k]);
     }
     for (; b < nblocks; b += G) {
+        uint8_t *slot = slots + (size_t)b * SLOT32;
 #pragma unroll
-        for (int k = 0; k < NROW; ++k) {
-            M[t + NT * k] = 0u;
+        for (int k = 0; k < NROW; ++k) {  // level 32, x-pass
             float r[32];
 #pragma unroll
             for (int i = 0; i < 32; ++i) r[i] = nx[k][i];
-            row<32, true, WT>(r);  // level 32, x-pass
+            row<32, true, WT>(r);
             lds_st<32>(A, xrow_lds_base<LAYOUT>(t + NT * k), 1, r);
         }
+        if (t < 2048 - NRUN32) cnt[NRUN32 + t] = 0u;
         __syncthreads();
-        if (b + G < nblocks) {  // prefetch the next block while this one is transformed
+        // Prefetch the next block's rows in two halves, each well under the
+        // 63 outstanding loads a wave may have (more would stall the wave
+        // until the first returns).  The last block reloads itself, so nx is
+        // always written and is dead between its use and its reload.
+        const int bn = min(b + G, nblocks - 1);
+        load_xrow_f32<LAYOUT>(src, g, bn, t, nx[0]);
 #pragma unroll
-            for (int k = 0; k < NROW; ++k) load_xrow_f32<LAYOUT>(src, g, b + G, t + NT * k, nx[k]);
+        for (int k = 0; k < NROW; ++k) {  // level 32, y-pass: row (z, x)
+            const int q = t + NT * k, z = q >> 5, x = q & 31;
+            float r[32];
+            lds_ld<32>(A, z * SP + x, RP, r);
+            row<32, true, WT>(r);
+            lds_st<32>(A, z * SP + x, RP, r);
         }
-        fwt32_yz_f32<WT>(A, t);
-        fwt16_8_f32<WT>(A, t);
-
-        // threshold: keep bits of this thread's stream segments (base 4^3 always kept)
-        float v[NROW][32];
-        uint32_t keep[NROW];
+        __syncthreads();
+        // level 32, z-pass: row (y, x) = physical mask word x + 32*y, bit z
+        // (level-32 layout (a0,a1,a2) = (y,x,z)).  Outside the 16^2 corner of
+        // (y,x) all 32 results are final; inside it only z >= 16 is, and the
+        // level-16 z-pass continues on z < 16.  Keep bits are taken here, in
+        // registers; the values stay in registers until they are written.
+        float r[NROW][32];
+        uint32_t kb[NROW];
 #pragma unroll
         for (int k = 0; k < NROW; ++k) {
-            const int sg = t + NT * k;
-            keep[k] = 0;
-            for_each_run(sg, [&](int q, int h, int addr, int stride, int word, int shift) {
-                uint32_t bits = 0;
+            const int q = t + NT * k, y = q >> 5, x = q & 31;
+            const bool corner = (y < 16) && (x < 16);
+            lds_ld<32>(A, y * RP + x, SP, r[k]);
+            row<32, true, WT>(r[k]);
+            uint32_t bits = 0;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    if (i < h) {
-                        const float c = A[addr + i * stride];
-                        v[k][q * h + i] = c;
-                        const bool kp = (sg < 2) || (fabsf(c) > eps);
-                        bits |= (uint32_t)kp << i;
-                    }
+            for (int z = 0; z < 32; ++z) bits |= (uint32_t)(fabsf(r[k][z]) > eps) << z;
+            if (corner) {
+                float c[16];
+#pragma unroll
+                for (int z = 0; z < 16; ++z) c[z] = r[k][z];
+                row<16, true, WT>(c);
+                lds_st<16>(A, y * RP + x, SP, c);
+                bits &= 0xffff0000u;
+            }
+            kb[k] = bits;
+            M[x + 32 * y] = bits;
+            // stream run (code c, i0 = y&15, i1 = x&15) of each z-half; code
+            // bits: z >= 16, x >= 16, y >= 16 (FullWaveletTransform.h:68-72)
+            const int c0 = 2 * (x >> 4) + 4 * (y >> 4);
+            const int base = (y & 15) * 16 + (x & 15);
+            if (!corner) cnt[(c0 - 1) * 256 + base] = __popc(bits & 0xffffu);
+            cnt[c0 * 256 + base] = __popc(bits >> 16);
+        }
+        __syncthreads();
+        {  // exclusive scan of the run counts in stream order
+            uint32_t c4[4], sum = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                c4[i] = cnt[4 * t + i];
+                sum += c4[i];
+            }
+            uint32_t S32;
+            uint32_t run = block_excl_scan(sum, wsum, &S32);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                cnt[4 * t + i] = run;
+                run += c4[i];
+            }
+            if (t == 0) s32_total = S32;
+        }
+        __syncthreads();
+        const uint32_t S32 = s32_total;
+        // level-32 survivors, from registers, zerobits applied.  The z >= 16
+        // half of A is dead after the z-pass (the remaining passes only touch
+        // the [0,16)^3 corner), so survivors are compacted there branch-free
+        // (unkept values go to a per-lane dummy word) and then copied out with
+        // 16-byte stores.  A block with more survivors than that half holds
+        // writes them straight to the slot instead.
+        uint32_t *d32 = reinterpret_cast<uint32_t *>(slot + MASK_BYTES) + CORNER_MAX;
+        const bool staged = S32 <= (uint32_t)STAGE32;
+        uint32_t *st32 = staged ? reinterpret_cast<uint32_t *>(A) + STAGE32_AT : d32;
+        const uint32_t dummy = (uint32_t)STAGE32 + (t & 63);
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) {
+            const int q = t + NT * k, y = q >> 5, x = q & 31;
+            const int c0 = 2 * (x >> 4) + 4 * (y >> 4);
+            const int base = (y & 15) * 16 + (x & 15);
+            uint32_t i0 = (c0 > 0) ? cnt[(c0 - 1) * 256 + base] : 0u;
+            uint32_t i1 = cnt[c0 * 256 + base];
+            if (staged) {
+#pragma unroll
+                for (int z = 0; z < 32; ++z) {
+                    const uint32_t bit = (kb[k] >> z) & 1u;
+                    uint32_t &i = z < 16 ? i0 : i1;
+                    st32[bit ? i : dummy] = __float_as_uint(r[k][z]) & zkeep;
+                    i += bit;
                 }
-                keep[k] |= bits << (q * h);
-                if (bits) atomicOr(&M[word], bits << shift);
-            });
-        }
-        // segment counts: low 16 bits = segment t, high 16 bits = segment t+NT
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan((uint32_t)__popc(keep[0]) | ((uint32_t)__popc(keep[1]) << 16), wsum, &tot);
-        const uint32_t S = (tot & 0xffffu) + (tot >> 16);
-        uint32_t first[NROW] = {ex & 0xffffu, (tot & 0xffffu) + (ex >> 16)};
-        __syncthreads();  // every thread has read A: reuse it as the survivor stage
-
-        uint32_t *stage = reinterpret_cast<uint32_t *>(A);
-        uint8_t *stage8 = reinterpret_cast<uint8_t *>(A);
+            } else {
 #pragma unroll
-        for (int k = 0; k < NROW; ++k) {
-            uint32_t idx = first[k];
-#pragma unroll
-            for (int j = 0; j < 32; ++j) {
-                if ((keep[k] >> j) & 1u) {
-                    const uint32_t w = __float_as_uint(v[k][j]) & zkeep;
-                    if (shuffle) {
-                        stage8[idx] = (uint8_t)w;
-                        stage8[S + idx] = (uint8_t)(w >> 8);
-                        stage8[2 * S + idx] = (uint8_t)(w >> 16);
-                        stage8[3 * S + idx] = (uint8_t)(w >> 24);
-                    } else {
-                        stage[idx] = w;
-                    }
-                    ++idx;
+                for (int z = 0; z < 32; ++z) {
+                    uint32_t &i = z < 16 ? i0 : i1;
+                    if ((kb[k] >> z) & 1u) d32[i++] = __float_as_uint(r[k][z]) & zkeep;
                 }
             }
         }
-        __syncthreads();
-        uint8_t *slot = slots + (size_t)b * SLOT32;
+        load_xrow_f32<LAYOUT>(src, g, bn, t + NT, nx[1]);  // second half of the prefetch
+        __syncthreads();  // stage complete, wsum free again
+        if (staged) {
+            const uint4 *s4 = reinterpret_cast<const uint4 *>(st32);
+            uint4 *d4 = reinterpret_cast<uint4 *>(d32);
+            for (uint32_t i = t; i < S32 / 4; i += NT) d4[i] = s4[i];
+            for (uint32_t i = (S32 & ~3u) + t; i < S32; i += NT) d32[i] = st32[i];
+        }
+        fwt16_8_f32<WT>(A, t);
+
+        // the [0,16)^3 corner: stream positions [0,4096) in pieces of 8 per
+        // thread (base 4^3 always kept); values re-read from A
+        uint32_t keep = 0;
+        float v[8];
+        corner_piece(t, [&](int q, int h, int addr, int stride, int word, int shift) {
+            uint32_t bits = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (i < h) {
+                    const float c = A[addr + i * stride];
+                    v[q * h + i] = c;
+                    bits |= (uint32_t)((t < 8) || (fabsf(c) > eps)) << i;
+                }
+            }
+            keep |= bits << (q * h);
+            atomicOr(&M[word], bits << shift);
+        });
+        uint32_t Sc;
+        uint32_t idx = block_excl_scan(__popc(keep), wsum, &Sc);
+        {
+            uint32_t *dc = reinterpret_cast<uint32_t *>(slot + MASK_BYTES);
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if ((keep >> j) & 1u) dc[idx++] = __float_as_uint(v[j]) & zkeep;
+        }
+        __syncthreads();  // mask complete
 #pragma unroll
         for (int k = 0; k < NROW; ++k) reinterpret_cast<uint32_t *>(slot)[t + NT * k] = M[t + NT * k];
-        uint32_t *dst = reinterpret_cast<uint32_t *>(slot + MASK_BYTES);
-        for (uint32_t i = t; i < S; i += NT) dst[i] = stage[i];
-        if (t == 0) nbytes[b] = MASK_BYTES + 4u * S;
-        __syncthreads();  // stage and M are read before the next block reuses them
+        if (t == 0) {
+            nbytes[b] = MASK_BYTES + 4u * (Sc + S32);
+            nbytes[nblocks + b] = Sc;
+        }
+        __syncthreads();  // A, M, cnt and wsum are read before the next block reuses them
     }
 }
 
@@ -456,30 +596,44 @@ __global__ __launch_bounds__(1024) void k_scan_records(const uint32_t *__restric
     if (t == 0) offsets[n] = tot;
 }
 
-// Copy each block's payload from its slot into the record stream, prefixed
-// by its int32 size word (SerializerIO_WaveletCompression_MPI_Simple.h:237-240).
+// Frame each block's payload into the record stream
+// (SerializerIO_WaveletCompression_MPI_Simple.h:237-240): size word, mask,
+// corner survivors then level-32 survivors, with the byte-plane shuffle
+// (WaveletCompressor.cpp:56-73: byte p of survivor i at p*S + i) if asked.
 __global__ __launch_bounds__(256) void k_pack_records(const uint8_t *__restrict__ slots, size_t slot_bytes,
-                                                      const uint32_t *__restrict__ nbytes,
+                                                      const uint32_t *__restrict__ nbytes, int nblocks, int shuffle,
                                                       const uint64_t *__restrict__ offsets,
                                                       uint8_t *__restrict__ records)
 {
     const int b = blockIdx.x, t = threadIdx.x;
-    const uint32_t n = nbytes[b];
-    const uint32_t nd = n / 4;
+    const uint32_t n = nbytes[b], sc = nbytes[nblocks + b];
+    const uint32_t S = (n - MASK_BYTES) / 4;
     uint32_t *dst = reinterpret_cast<uint32_t *>(records + offsets[b]);
+    uint8_t *dst8 = records + offsets[b] + 4 + MASK_BYTES;
     const uint32_t *src = reinterpret_cast<const uint32_t *>(slots + (size_t)b * slot_bytes);
+    const uint32_t *corner = src + MASK_BYTES / 4, *l32 = corner + CORNER_MAX;
     if (t == 0) dst[0] = n;
-    for (uint32_t base = t; base < nd; base += 256 * 4) {
+    for (uint32_t i = t; i < MASK_BYTES / 4; i += 256) dst[1 + i] = src[i];
+    for (uint32_t base = t; base < S; base += 256 * 4) {
         uint32_t v[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t i = base + 256 * k;
-            v[k] = i < nd ? src[i] : 0u;
+            v[k] = i < S ? (i < sc ? corner[i] : l32[i - sc]) : 0u;
         }
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const uint32_t i = base + 256 * k;
-            if (i < nd) dst[1 + i] = v[k];
+            if (i < S) {
+                if (shuffle) {
+                    dst8[i] = (uint8_t)v[k];
+                    dst8[S + i] = (uint8_t)(v[k] >> 8);
+                    dst8[2 * S + i] = (uint8_t)(v[k] >> 16);
+                    dst8[3 * S + i] = (uint8_t)(v[k] >> 24);
+                } else {
+                    dst[1 + MASK_BYTES / 4 + i] = v[k];
+                }
+            }
         }
     }
 }
@@ -520,16 +674,145 @@ CZW_DI void fetch_record(const uint8_t *__restrict__ records, uint64_t rec_bytes
     }
 }
 
+// Undo the byte-plane shuffle of S staged survivors in place (byte p of
+// survivor i at byte p*S + i, WaveletCompressor.cpp:75-93): each thread
+// gathers its survivors i = t + NT*j into registers, then all write back.
+template <int MAXPT>
+CZW_DI void unshuffle_stage(uint32_t *stage, uint32_t S)
+{
+    const int t = threadIdx.x;
+    const uint8_t *b8 = reinterpret_cast<const uint8_t *>(stage);
+    uint32_t u[MAXPT];
+#pragma unroll
+    for (int j = 0; j < MAXPT; ++j) {
+        const uint32_t i = (uint32_t)(t + NT * j);
+        u[j] = i < S ? ((uint32_t)b8[i] | ((uint32_t)b8[S + i] << 8) | ((uint32_t)b8[2 * S + i] << 16) |
+                        ((uint32_t)b8[3 * S + i] << 24))
+                     : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < MAXPT; ++j) {
+        const uint32_t i = (uint32_t)(t + NT * j);
+        if (i < S) stage[i] = u[j];
+    }
+    __syncthreads();
+}
+
+// Decompression of one block whose survivors fit the z >= 16 half of A
+// (S <= STAGE32): survivors are staged there; the corner positions are
+// written into A branch-free (value or +0); the level-32 survivors are
+// gathered straight into registers for the fused level-16/level-32 z-pass.
+// Returns false (uniformly) when the record is inconsistent.
+template <int WT>
+CZW_DI bool decode_block_fast_f32(float *A, uint32_t *M, uint32_t *cnt, uint32_t *wsum, uint32_t *flag,
+                                  const RecPrefetch &cur, const uint32_t *pay, uint32_t S, int shuffle,
+                                  float (&r)[NROW][32])
+{
+    const int t = threadIdx.x;
+    uint32_t *stage = reinterpret_cast<uint32_t *>(A) + STAGE32_AT;
+    const uint8_t *stage8 = reinterpret_cast<const uint8_t *>(stage);
+    if (t == 0) *flag = 1u;
+#pragma unroll
+    for (int k = 0; k < NROW; ++k) M[t + NT * k] = cur.mw[k];
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+        const uint32_t i = (uint32_t)(k * NT + t);
+        if (i < S) stage[i] = cur.sv[k];
+    }
+    for (uint32_t i = PF * NT + t; i < S; i += NT) stage[i] = pay[MASK_BYTES / 4 + i];
+    if (t < 64) stage[STAGE32 + t] = 0u;  // the dummy words read for dropped coefficients
+    __syncthreads();
+    if (shuffle) unshuffle_stage<(STAGE32 + NT - 1) / NT>(stage, S);
+    // the 4^3 base is always stored (FullWaveletTransform.h:163-176): its 64
+    // mask bits are the low nibbles of words a1 + 32*a0, a0, a1 < 4
+    if (t < 16 && (M[(t & 3) + 32 * (t >> 2)] & 0xfu) != 0xfu) *flag = 0u;
+    // counts in stream order: corner piece t at cnt[t], level-32 run rho at cnt[512 + rho]
+    uint32_t keep = 0;
+    corner_piece(t, [&](int q, int h, int, int, int word, int shift) {
+        const uint32_t hm = (1u << h) - 1u;
+        keep |= ((t < 8) ? hm : ((M[word] >> shift) & hm)) << (q * h);
+    });
+    cnt[t] = __popc(keep);
+    uint32_t mw[NROW];
+#pragma unroll
+    for (int k = 0; k < NROW; ++k) {
+        const int q = t + NT * k, y = q >> 5, x = q & 31;
+        const bool corner = (y < 16) && (x < 16);
+        mw[k] = M[x + 32 * y] & (corner ? 0xffff0000u : 0xffffffffu);
+        const int c0 = 2 * (x >> 4) + 4 * (y >> 4), base = (y & 15) * 16 + (x & 15);
+        if (!corner) cnt[512 + (c0 - 1) * 256 + base] = __popc(mw[k] & 0xffffu);
+        cnt[512 + c0 * 256 + base] = __popc(mw[k] >> 16);
+    }
+    if (t < 2560 - 512 - NRUN32) cnt[512 + NRUN32 + t] = 0u;
+    __syncthreads();
+    {  // exclusive scan of the 2560 counts, 5 consecutive per thread
+        uint32_t c5[5], sum = 0;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            c5[i] = cnt[5 * t + i];
+            sum += c5[i];
+        }
+        uint32_t tot;
+        uint32_t run = block_excl_scan(sum, wsum, &tot);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            cnt[5 * t + i] = run;
+            run += c5[i];
+        }
+        if (tot != S) *flag = 0u;
+    }
+    __syncthreads();
+    if (!*flag) return false;
+    // the 64 dummy words after the stage hold +0: a dropped coefficient reads
+    // its lane's dummy word
+    auto sv = [&](uint32_t i) -> float { return __uint_as_float(stage[i]); };
+    const uint32_t dummy = (uint32_t)STAGE32 + (t & 63);
+    {  // corner positions: survivor or +0
+        uint32_t idx = cnt[t];
+        corner_piece(t, [&](int q, int h, int addr, int stride, int, int) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (i < h) {
+                    const uint32_t bit = (keep >> (q * h + i)) & 1u;
+                    const float v = sv(bit ? idx : dummy);
+                    A[addr + i * stride] = v;
+                    idx += bit;
+                }
+            }
+        });
+    }
+    // level-32 survivors of this thread's z-columns, straight into registers
+#pragma unroll
+    for (int k = 0; k < NROW; ++k) {
+        const int q = t + NT * k, y = q >> 5, x = q & 31;
+        const int c0 = 2 * (x >> 4) + 4 * (y >> 4), base = (y & 15) * 16 + (x & 15);
+        uint32_t i0 = (c0 > 0) ? cnt[512 + (c0 - 1) * 256 + base] : 0u;
+        uint32_t i1 = cnt[512 + c0 * 256 + base];
+#pragma unroll
+        for (int z = 0; z < 32; ++z) {
+            const uint32_t bit = (mw[k] >> z) & 1u;
+            uint32_t &i = z < 16 ? i0 : i1;
+            const float v = sv(bit ? i : dummy);
+            r[k][z] = v;
+            i += bit;
+        }
+    }
+    __syncthreads();  // stage consumed; corner of A complete
+    return true;
+}
+
 template <int WT, int LAYOUT>
 __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict__ records, uint64_t rec_bytes,
                                                        const uint64_t *__restrict__ offsets, int nblocks,
                                                        int shuffle, float *__restrict__ dst, FieldGeom g,
                                                        uint32_t *__restrict__ err)
 {
-    __shared__ float A[LDS_WORDS];
+    __shared__ __align__(16) float A[LDS_WORDS];
     __shared__ uint32_t M[1024];
+    __shared__ uint32_t cnt[2560];
     __shared__ uint32_t wsum[NT / 64];
-    __shared__ uint32_t coarse_ok;
+    __shared__ uint32_t flag;
     const int t = threadIdx.x;
     const int G = gridDim.x;
     int b = blockIdx.x;
@@ -551,80 +834,131 @@ __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict
         }
         const uint32_t S = (nb - MASK_BYTES) >> 2;
         const uint32_t *pay = reinterpret_cast<const uint32_t *>(records + ob_cur + 4);
-        __syncthreads();  // the previous block's last reads of A, M and coarse_ok are done
-        if (t == 0) coarse_ok = 1u;
-#pragma unroll
-        for (int k = 0; k < NROW; ++k) M[t + NT * k] = cur.mw[k];
-        uint32_t *stage = reinterpret_cast<uint32_t *>(A);
-        const uint8_t *stage8 = reinterpret_cast<const uint8_t *>(A);
-#pragma unroll
-        for (int k = 0; k < PF; ++k) {
-            const uint32_t i = (uint32_t)(k * NT + t);
-            if (i < S) stage[i] = cur.sv[k];
-        }
-        for (uint32_t i = PF * NT + t; i < S; i += NT) stage[i] = pay[MASK_BYTES / 4 + i];
-        __syncthreads();
-        // the 4^3 base is always stored (FullWaveletTransform.h:163-176): its 64
-        // mask bits are the low nibbles of words a1 + 32*a0, a0, a1 < 4
-        if (t < 16 && (M[(t & 3) + 32 * (t >> 2)] & 0xfu) != 0xfu) coarse_ok = 0u;
-
-        uint32_t keep[NROW];
-#pragma unroll
-        for (int k = 0; k < NROW; ++k) {
-            const int sg = t + NT * k;
-            keep[k] = 0;
-            for_each_run(sg, [&](int q, int h, int, int, int word, int shift) {
-                const uint32_t hm = (h == 16) ? 0xffffu : ((1u << h) - 1u);
-                const uint32_t bits = (sg < 2) ? hm : ((M[word] >> shift) & hm);
-                keep[k] |= bits << (q * h);
-            });
-        }
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan((uint32_t)__popc(keep[0]) | ((uint32_t)__popc(keep[1]) << 16), wsum, &tot);
-        // with all base bits set, the kept count equals the mask popcount
-        // that WaveletCompressor.cpp:235-251 checks against the payload size
-        if ((tot & 0xffffu) + (tot >> 16) != S || !coarse_ok) {
-            if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_POPCOUNT);
-            continue;
-        }
-        const uint32_t first[NROW] = {ex & 0xffffu, (tot & 0xffffu) + (ex >> 16)};
-        float v[NROW][32];
-#pragma unroll
-        for (int k = 0; k < NROW; ++k) {
-            uint32_t idx = first[k];
-#pragma unroll
-            for (int j = 0; j < 32; ++j) {
-                float c = 0.0f;
-                if ((keep[k] >> j) & 1u) {
-                    uint32_t w;
-                    if (shuffle)
-                        w = (uint32_t)stage8[idx] | ((uint32_t)stage8[S + idx] << 8) |
-                            ((uint32_t)stage8[2 * S + idx] << 16) | ((uint32_t)stage8[3 * S + idx] << 24);
-                    else
-                        w = stage[idx];
-                    c = __uint_as_float(w);
-                    ++idx;
-                }
-                v[k][j] = c;
+        __syncthreads();  // the previous block's last reads of A, M, cnt and flag are done
+        float r[NROW][32];
+        if (S <= (uint32_t)STAGE32) {
+            if (!decode_block_fast_f32<WT>(A, M, cnt, wsum, &flag, cur, pay, S, shuffle, r)) {
+                if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_POPCOUNT);
+                continue;
             }
+            iwt16_8_f32<WT>(A, t);
+#pragma unroll
+            for (int k = 0; k < NROW; ++k) {  // level 16 z-pass (corner columns), then level 32 z-pass
+                const int q = t + NT * k, y = q >> 5, x = q & 31;
+                if (y < 16 && x < 16) {
+                    float c[16];
+                    lds_ld<16>(A, y * RP + x, SP, c);
+                    row<16, false, WT>(c);
+#pragma unroll
+                    for (int z = 0; z < 16; ++z) r[k][z] = c[z];
+                }
+                row<32, false, WT>(r[k]);
+                lds_st<32>(A, y * RP + x, SP, r[k]);
+            }
+            __syncthreads();
+        } else {
+            // dense block: survivors staged over A, every position gathered
+            // into registers, then placed; the full inverse follows
+            if (t == 0) flag = 1u;
+#pragma unroll
+            for (int k = 0; k < NROW; ++k) M[t + NT * k] = cur.mw[k];
+            uint32_t *stage = reinterpret_cast<uint32_t *>(A);
+#pragma unroll
+            for (int k = 0; k < PF; ++k) stage[k * NT + t] = cur.sv[k];
+            for (uint32_t i = PF * NT + t; i < S; i += NT) stage[i] = pay[MASK_BYTES / 4 + i];
+            __syncthreads();
+            if (t < 16 && (M[(t & 3) + 32 * (t >> 2)] & 0xfu) != 0xfu) flag = 0u;
+            uint32_t keep[NROW];
+#pragma unroll
+            for (int k = 0; k < NROW; ++k) {
+                const int sg = t + NT * k;
+                keep[k] = 0;
+                for_each_run(sg, [&](int q, int h, int, int, int word, int shift) {
+                    const uint32_t hm = (h == 16) ? 0xffffu : ((1u << h) - 1u);
+                    const uint32_t bits = (sg < 2) ? hm : ((M[word] >> shift) & hm);
+                    keep[k] |= bits << (q * h);
+                });
+            }
+            uint32_t tot;
+            const uint32_t ex =
+                block_excl_scan((uint32_t)__popc(keep[0]) | ((uint32_t)__popc(keep[1]) << 16), wsum, &tot);
+            // with all base bits set, the kept count equals the mask popcount
+            // that WaveletCompressor.cpp:235-251 checks against the payload size
+            if ((tot & 0xffffu) + (tot >> 16) != S || !flag) {
+                if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_POPCOUNT);
+                continue;
+            }
+            const uint32_t first[NROW] = {ex & 0xffffu, (tot & 0xffffu) + (ex >> 16)};
+#pragma unroll
+            for (int k = 0; k < NROW; ++k) {
+                uint32_t idx = first[k];
+#pragma unroll
+                for (int j = 0; j < 32; ++j) {
+                    float c = 0.0f;
+                    if ((keep[k] >> j) & 1u) {
+                        const uint8_t *b8 = reinterpret_cast<const uint8_t *>(stage);
+                        c = __uint_as_float(shuffle ? ((uint32_t)b8[idx] | ((uint32_t)b8[S + idx] << 8) |
+                                                       ((uint32_t)b8[2 * S + idx] << 16) |
+                                                       ((uint32_t)b8[3 * S + idx] << 24))
+                                                    : stage[idx]);
+                        ++idx;
+                    }
+                    r[k][j] = c;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < NROW; ++k) {
+                for_each_run(t + NT * k, [&](int q, int h, int addr, int stride, int, int) {
+#pragma unroll
+                    for (int i = 0; i < 16; ++i)
+                        if (i < h) A[addr + i * stride] = r[k][q * h + i];
+                });
+            }
+            __syncthreads();
+            iwt_most_f32<WT>(A, t);
         }
-        __syncthreads();  // all survivors gathered: A becomes the coefficient array
+        if (S <= (uint32_t)STAGE32) {
 #pragma unroll
-        for (int k = 0; k < NROW; ++k) {
-            for_each_run(t + NT * k, [&](int q, int h, int addr, int stride, int, int) {
-#pragma unroll
-                for (int i = 0; i < 16; ++i)
-                    if (i < h) A[addr + i * stride] = v[k][q * h + i];
-            });
+            for (int k = 0; k < NROW; ++k) {  // level 32 y-pass
+                const int q = t + NT * k, z = q >> 5, x = q & 31;
+                float rr[32];
+                lds_ld<32>(A, z * SP + x, RP, rr);
+                row<32, false, WT>(rr);
+                lds_st<32>(A, z * SP + x, RP, rr);
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        iwt_most_f32<WT>(A, t);
+        if constexpr (LAYOUT == LAYOUT_BLOCKS) {
 #pragma unroll
-        for (int k = 0; k < NROW; ++k) {
-            float r[32];
-            lds_ld<32>(A, xrow_lds_base<LAYOUT>(t + NT * k), 1, r);
-            row<32, false, WT>(r);  // level 32, x-pass
-            store_xrow_f32<LAYOUT>(dst, g, b, t + NT * k, r);
+            for (int k = 0; k < NROW; ++k) {
+                float rr[32];
+                lds_ld<32>(A, xrow_lds_base<LAYOUT>(t + NT * k), 1, rr);
+                row<32, false, WT>(rr);  // level 32, x-pass
+                store_xrow_f32<LAYOUT>(dst, g, b, t + NT * k, rr);
+            }
+        } else {
+            // level 32 x-pass back into A, then 16-byte stores along the
+            // field's contiguous Z: four times fewer store instructions than
+            // storing each x-row element on its own
+#pragma unroll
+            for (int k = 0; k < NROW; ++k) {
+                float rr[32];
+                const int base = (t & 31) * SP + ((t + NT * k) >> 5) * RP;
+                lds_ld<32>(A, base, 1, rr);
+                row<32, false, WT>(rr);
+                lds_st<32>(A, base, 1, rr);
+            }
+            __syncthreads();
+            size_t xs;
+            float *p0 = dst + field_row(g, b, 0, 0, &xs);
+#pragma unroll 4
+            for (int i = t; i < 32 * 32 * 8; i += NT) {  // (x, y, z4) with z4 fastest
+                const int z4 = i & 7, y = (i >> 3) & 31, x = i >> 8;
+                const int a = (4 * z4) * SP + y * RP + x;
+                const float4 v = make_float4(A[a], A[a + SP], A[a + 2 * SP], A[a + 3 * SP]);
+                *reinterpret_cast<float4 *>(p0 + x * xs + (size_t)y * g.nz + 4 * z4) = v;
+            }
         }
     }
 }
@@ -664,7 +998,7 @@ static bool zb_ok(int zb) { return zb == 0 || zb == 4 || zb == 8 || zb == 12 || 
 
 template <int LAYOUT>
 static int compress_slots_f32(const float *src, int nblocks, FieldGeom g, float eps, int wtype, int zerobits,
-                              int shuffle, uint8_t *slots, uint32_t *nb, hipStream_t s)
+                              uint8_t *slots, uint32_t *nb, hipStream_t s)
 {
     const int wt = wt_index(wtype);
     if (wt < 0 || !zb_ok(zerobits) || nblocks <= 0 || !src || !slots || !nb) return CZW_EINVAL;
@@ -672,18 +1006,19 @@ static int compress_slots_f32(const float *src, int nblocks, FieldGeom g, float 
     const void *fn = wt == 1 ? (const void *)k_compress_f32<1, LAYOUT>
                    : wt == 2 ? (const void *)k_compress_f32<2, LAYOUT> : (const void *)k_compress_f32<3, LAYOUT>;
     const dim3 grid(std::min(nblocks, resident_groups(fn, NT))), blk(NT);
-    if (wt == 1) hipLaunchKernelGGL((k_compress_f32<1, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, shuffle, slots, nb);
-    if (wt == 2) hipLaunchKernelGGL((k_compress_f32<2, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, shuffle, slots, nb);
-    if (wt == 3) hipLaunchKernelGGL((k_compress_f32<3, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, shuffle, slots, nb);
+    if (wt == 1) hipLaunchKernelGGL((k_compress_f32<1, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb);
+    if (wt == 2) hipLaunchKernelGGL((k_compress_f32<2, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb);
+    if (wt == 3) hipLaunchKernelGGL((k_compress_f32<3, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb);
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
 
-static int pack_records(const uint8_t *slots, size_t slot_bytes, const uint32_t *nb, int nblocks, uint8_t *records,
-                        uint64_t *offsets, hipStream_t s)
+static int pack_records(const uint8_t *slots, size_t slot_bytes, const uint32_t *nb, int nblocks, int shuffle,
+                        uint8_t *records, uint64_t *offsets, hipStream_t s)
 {
     if (nblocks <= 0 || !slots || !nb || !records || !offsets || (slot_bytes & 3)) return CZW_EINVAL;
     hipLaunchKernelGGL(k_scan_records, dim3(1), dim3(1024), 0, s, nb, nblocks, offsets);
-    hipLaunchKernelGGL(k_pack_records, dim3(nblocks), dim3(256), 0, s, slots, slot_bytes, nb, offsets, records);
+    hipLaunchKernelGGL(k_pack_records, dim3(nblocks), dim3(256), 0, s, slots, slot_bytes, nb, nblocks, shuffle, offsets,
+                       records);
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
 
@@ -694,9 +1029,9 @@ static int compress_f32(const float *src, int nblocks, FieldGeom g, float eps, i
     if (!records || !offsets || !workspace || nblocks <= 0) return CZW_EINVAL;
     uint8_t *slots = static_cast<uint8_t *>(workspace);
     uint32_t *nb = reinterpret_cast<uint32_t *>(slots + SLOT32 * (size_t)nblocks);
-    const int rc = compress_slots_f32<LAYOUT>(src, nblocks, g, eps, wtype, zerobits, shuffle, slots, nb, s);
+    const int rc = compress_slots_f32<LAYOUT>(src, nblocks, g, eps, wtype, zerobits, slots, nb, s);
     if (rc != CZW_OK) return rc;
-    return pack_records(slots, SLOT32, nb, nblocks, static_cast<uint8_t *>(records), offsets, s);
+    return pack_records(slots, SLOT32, nb, nblocks, shuffle, static_cast<uint8_t *>(records), offsets, s);
 }
 
 template <int LAYOUT>
@@ -733,7 +1068,7 @@ size_t czw_compress_workspace_bytes(int nblocks, int prec)
 {
     if (nblocks <= 0 || (prec != 4 && prec != 8)) return 0;
     const size_t slot = MASK_BYTES + (size_t)prec * NB3;
-    return slot * (size_t)nblocks + 4 * (size_t)nblocks;
+    return slot * (size_t)nblocks + 8 * (size_t)nblocks;
 }
 
 size_t czw_records_bound(int nblocks, int prec)
@@ -755,21 +1090,21 @@ int czw_compress_field(const void *field, int nx, int ny, int nz, int prec, floa
 }
 
 int czw_compress_field_slots(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype,
-                             int zerobits, int shuffle, void *slots, uint32_t *nbytes, hipStream_t stream)
+                             int zerobits, void *slots, uint32_t *nbytes, hipStream_t stream)
 {
     if (!field_ok(nx, ny, nz)) return CZW_EINVAL;
     const int nb = (nx / 32) * (ny / 32) * (nz / 32);
     const FieldGeom g{nx, ny, nz};
     if (prec == 4)
         return compress_slots_f32<LAYOUT_FIELD>(static_cast<const float *>(field), nb, g, eps, wtype, zerobits,
-                                                shuffle, static_cast<uint8_t *>(slots), nbytes, stream);
+                                                static_cast<uint8_t *>(slots), nbytes, stream);
     return prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
 }
 
-int czw_pack_records(const void *slots, size_t slot_bytes, const uint32_t *nbytes, int nblocks, void *records,
-                     uint64_t *offsets, hipStream_t stream)
+int czw_pack_records(const void *slots, size_t slot_bytes, const uint32_t *nbytes, int nblocks, int shuffle,
+                     void *records, uint64_t *offsets, hipStream_t stream)
 {
-    return pack_records(static_cast<const uint8_t *>(slots), slot_bytes, nbytes, nblocks,
+    return pack_records(static_cast<const uint8_t *>(slots), slot_bytes, nbytes, nblocks, shuffle,
                         static_cast<uint8_t *>(records), offsets, stream);
 }
 

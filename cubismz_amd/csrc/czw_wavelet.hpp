@@ -23,80 +23,77 @@ namespace czw {
 
 template <typename R> struct Pred;
 
+// Stencil constants of the binary32 path, read from constant memory so that
+// they live in scalar registers: the fused multiply-adds can then use the
+// three-address v_fma_f64 form instead of copying the shared addend.
+// Values are exactly the reference's binary64 literals.
+enum {
+    K_0P0625 = 0, K_M0P0625, K_0P3125, K_M0P3125, K_0P5625, K_0P9375, K_1P3125, K_M2P1875, K_2P1875,
+    K_0P125, K_M0P125, K_0P5, K_M0P5, K_0P625, K_1P375, K_COUNT
+};
+__constant__ double czw_kc[K_COUNT] = {0.0625, -0.0625, 0.3125, -0.3125, 0.5625, 0.9375, 1.3125, -2.1875, 2.1875,
+                                       0.125,  -0.125,  0.5,    -0.5,    0.625,  1.375};
+
 template <> struct Pred<float> {
     typedef float R;
+    static CZW_DI double K(int i) { return czw_kc[i]; }
     static CZW_DI float rnd(double t) { return (float)t; }
+    static CZW_DI double F(double k, R x, double t) { return __builtin_fma(k, (double)x, t); }
     // wtype 1/2: WaveletsOnInterval.h:33-51
     static CZW_DI float p4_first(R a, R b, R c, R e)
     {
-        double t = 0.3125 * (double)a;
-        t = __builtin_fma(0.9375, (double)b, t);
-        t = __builtin_fma(-0.3125, (double)c, t);
-        t = __builtin_fma(0.0625, (double)e, t);
-        return rnd(t);
+        double t = K(K_0P3125) * (double)a;
+        t = F(K(K_0P9375), b, t);
+        t = F(K(K_M0P3125), c, t);
+        return rnd(F(K(K_0P0625), e, t));
     }
     static CZW_DI float p4_mid(R a, R b, R c, R e)
     {
-        double t = -0.0625 * (double)a;
-        t = __builtin_fma(0.5625, (double)b, t);
-        t = __builtin_fma(0.5625, (double)c, t);
-        t = __builtin_fma(-0.0625, (double)e, t);
-        return rnd(t);
+        double t = K(K_M0P0625) * (double)a;
+        t = F(K(K_0P5625), b, t);
+        t = F(K(K_0P5625), c, t);
+        return rnd(F(K(K_M0P0625), e, t));
     }
     static CZW_DI float p4_otl(R a, R b, R c, R e)
     {
-        double t = 0.0625 * (double)a;
-        t = __builtin_fma(-0.3125, (double)b, t);
-        t = __builtin_fma(0.9375, (double)c, t);
-        t = __builtin_fma(0.3125, (double)e, t);
-        return rnd(t);
+        double t = K(K_0P0625) * (double)a;
+        t = F(K(K_M0P3125), b, t);
+        t = F(K(K_0P9375), c, t);
+        return rnd(F(K(K_0P3125), e, t));
     }
     static CZW_DI float p4_last(R a, R b, R c, R e)
     {
-        double t = -0.3125 * (double)a;
-        t = __builtin_fma(1.3125, (double)b, t);
-        t = __builtin_fma(-2.1875, (double)c, t);
-        t = __builtin_fma(2.1875, (double)e, t);
-        return rnd(t);
+        double t = K(K_M0P3125) * (double)a;
+        t = F(K(K_1P3125), b, t);
+        t = F(K(K_M2P1875), c, t);
+        return rnd(F(K(K_2P1875), e, t));
     }
     // wtype 3: WaveletsOnInterval.h:55-83
     static CZW_DI float q0_first(R a, R b, R c)
     {
-        double t = 1.375 * (double)a;
-        t = __builtin_fma(-0.5, (double)b, t);
-        return rnd(__builtin_fma(0.125, (double)c, t));
+        const double t = F(K(K_M0P5), b, K(K_1P375) * (double)a);
+        return rnd(F(K(K_0P125), c, t));
     }
     static CZW_DI float q1_first(R a, R b, R c)
     {
-        double t = 0.625 * (double)a;
-        t = __builtin_fma(0.5, (double)b, t);
-        return rnd(__builtin_fma(-0.125, (double)c, t));
+        const double t = F(K(K_0P5), b, K(K_0P625) * (double)a);
+        return rnd(F(K(K_M0P125), c, t));
     }
-    static CZW_DI float q0_mid(R a, R b, R c)
-    {
-        const double t = __builtin_fma(0.125, (double)a, (double)b);
-        return rnd(__builtin_fma(-0.125, (double)c, t));
-    }
-    static CZW_DI float q1_mid(R a, R b, R c)
-    {
-        const double t = __builtin_fma(-0.125, (double)a, (double)b);
-        return rnd(__builtin_fma(0.125, (double)c, t));
-    }
+    static CZW_DI float q0_mid(R a, R b, R c) { return rnd(F(K(K_M0P125), c, F(K(K_0P125), a, (double)b))); }
+    static CZW_DI float q1_mid(R a, R b, R c) { return rnd(F(K(K_0P125), c, F(K(K_M0P125), a, (double)b))); }
     static CZW_DI float q0_last(R a, R b, R c)
     {
-        double t = -0.125 * (double)a;
-        t = __builtin_fma(0.5, (double)b, t);
-        return rnd(__builtin_fma(0.625, (double)c, t));
+        const double t = F(K(K_0P5), b, K(K_M0P125) * (double)a);
+        return rnd(F(K(K_0P625), c, t));
     }
     static CZW_DI float q1_last(R a, R b, R c)
     {
-        double t = 0.125 * (double)a;
-        t = __builtin_fma(-0.5, (double)b, t);
-        return rnd(__builtin_fma(1.375, (double)c, t));
+        const double t = F(K(K_M0P5), b, K(K_0P125) * (double)a);
+        return rnd(F(K(K_1P375), c, t));
     }
     // wtype 2 lifting update s +- 0.5*d in binary64 (WaveletsOnInterval.h:124-126,137-139)
-    static CZW_DI float lift_add(R s, R d) { return rnd(__builtin_fma(0.5, (double)d, (double)s)); }
-    static CZW_DI float lift_sub(R s, R d) { return rnd(__builtin_fma(-0.5, (double)d, (double)s)); }
+    static CZW_DI float lift_add(R s, R d) { return rnd(F(K(K_0P5), d, (double)s)); }
+    static CZW_DI float lift_sub(R s, R d) { return rnd(F(K(K_M0P5), d, (double)s)); }
 };
 
 template <> struct Pred<double> {

@@ -56,9 +56,9 @@ def load_library():
     L.czw_slot_bytes.restype = sz
     L.czw_slot_bytes.argtypes = [i]
     L.czw_compress_field_slots.restype = i
-    L.czw_compress_field_slots.argtypes = [vp, i, i, i, i, f, i, i, i, vp, vp, vp]
+    L.czw_compress_field_slots.argtypes = [vp, i, i, i, i, f, i, i, vp, vp, vp]
     L.czw_pack_records.restype = i
-    L.czw_pack_records.argtypes = [vp, sz, vp, i, vp, vp, vp]
+    L.czw_pack_records.argtypes = [vp, sz, vp, i, i, vp, vp, vp]
     L.czw_compress_blocks.restype = i
     L.czw_compress_blocks.argtypes = [vp, i, i, f, i, i, i, vp, vp, vp, vp]
     L.czw_decompress_field.restype = i
@@ -137,23 +137,24 @@ def compress_field(field, eps, wtype=3, zerobits=0, shuffle=False, records=None,
     return records, offsets
 
 
-def compress_field_slots(field, eps, wtype, zerobits, shuffle, slots, nbytes):
-    """Stage 1 of compress_field: per-block payloads into fixed-size slots."""
+def compress_field_slots(field, eps, wtype, zerobits, slots, nbytes):
+    """Stage 1 of compress_field: per-block payloads into fixed-size slots
+    (nbytes: 2*nblocks uint32, see czw.h)."""
     torch = _torch()
     L = load_library()
     nx, ny, nz = (int(s) for s in field.shape)
     rc = L.czw_compress_field_slots(C.c_void_p(field.data_ptr()), nx, ny, nz, _prec(field.dtype, torch), float(eps),
-                                    int(wtype), int(zerobits), int(bool(shuffle)), C.c_void_p(slots.data_ptr()),
+                                    int(wtype), int(zerobits), C.c_void_p(slots.data_ptr()),
                                     C.c_void_p(nbytes.data_ptr()), _stream(torch))
     _check(rc, "czw_compress_field_slots")
 
 
-def pack_records(slots, slot_bytes, nbytes, nblocks, records, offsets):
+def pack_records(slots, slot_bytes, nbytes, nblocks, shuffle, records, offsets):
     """Stage 2 of compress_field: frame slot payloads into the record stream."""
     torch = _torch()
     rc = load_library().czw_pack_records(C.c_void_p(slots.data_ptr()), int(slot_bytes), C.c_void_p(nbytes.data_ptr()),
-                                         int(nblocks), C.c_void_p(records.data_ptr()), C.c_void_p(offsets.data_ptr()),
-                                         _stream(torch))
+                                         int(nblocks), int(bool(shuffle)), C.c_void_p(records.data_ptr()),
+                                         C.c_void_p(offsets.data_ptr()), _stream(torch))
     _check(rc, "czw_pack_records")
 
 

@@ -84,16 +84,18 @@ int czw_compress_field(const void *field, int nx, int ny, int nz, int prec, floa
                        int zerobits, int shuffle, void *records, uint64_t *offsets, void *workspace,
                        hipStream_t stream);
 
-/* The two stages of czw_compress_field, for callers that want fixed-size
- * per-block slots or time the stages apart.  Stage 1 writes block b's payload
- * to slots + b*czw_slot_bytes(prec) and its size to nbytes[b]; stage 2 frames
- * the payloads into the contiguous record stream (exclusive scan of
- * 4 + nbytes[b] into offsets, then a copy). */
+/* The two stages of czw_compress_field, for callers that time the stages
+ * apart or keep fixed-size per-block slots.  Stage 1 writes block b's mask and
+ * survivors (zerobits applied, not yet shuffled) into slots +
+ * b*czw_slot_bytes(prec) in an internal split layout, and sets nbytes[b] to
+ * the payload size and nbytes[nblocks+b] to an internal split count (nbytes
+ * has 2*nblocks entries).  Stage 2 frames the payloads into the contiguous
+ * record stream, applying the byte shuffle when shuffle != 0. */
 size_t czw_slot_bytes(int prec);
 int czw_compress_field_slots(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype,
-                             int zerobits, int shuffle, void *slots, uint32_t *nbytes, hipStream_t stream);
-int czw_pack_records(const void *slots, size_t slot_bytes, const uint32_t *nbytes, int nblocks, void *records,
-                     uint64_t *offsets, hipStream_t stream);
+                             int zerobits, void *slots, uint32_t *nbytes, hipStream_t stream);
+int czw_pack_records(const void *slots, size_t slot_bytes, const uint32_t *nbytes, int nblocks, int shuffle,
+                     void *records, uint64_t *offsets, hipStream_t stream);
 
 /* Same as czw_compress_field for blocks already laid out [nb][z][y][x]. */
 int czw_compress_blocks(const void *blocks, int nblocks, int prec, float eps, int wtype, int zerobits,

@@ -119,3 +119,27 @@ def test_against_compiled_reference(torch_gpu, oracle, wtype):
         ix, iy, iz = b % 2, (b // 2) % 3, b // 6
         blk = np.ascontiguousarray(fn[32 * ix:32 * ix + 32, 32 * iy:32 * iy + 32, 32 * iz:32 * iz + 32].transpose(2, 1, 0))
         assert got[b] == oracle.ref_compress(blk, 1e-3, wtype, 4, True).tobytes()
+
+
+@pytest.mark.parametrize("wtype", [1, 3])
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_dense_and_mixed_blocks(torch_gpu, oracle, wtype, shuffle):
+    """Blocks whose survivors overflow the on-chip stages take the dense code
+    paths of both kernels; a field mixing dense noise and smooth regions
+    exercises both paths in one launch."""
+    torch = torch_gpu
+    print(f"[case] dense wtype={wtype} sh={shuffle}", flush=True)
+    rng = np.random.default_rng(17)
+    f = fields.bumps_field(SHAPE, seed=5).numpy().copy()
+    f[:32] += rng.standard_normal((32,) + SHAPE[1:]).astype(np.float32)  # blocks with ix = 0 become dense
+    ft = torch.from_numpy(f)
+    for eps in (1e-4, 0.0):
+        rec, off = czw.compress_field(ft.cuda(), eps, wtype, 4, shuffle)
+        got = _payloads(rec, off)
+        slots, nbytes = oracle.compress_field(f, eps, wtype, 4, shuffle)
+        assert max(nbytes) > 4096 + 4 * 16848  # at least one block past the stage capacity
+        for b, p in enumerate(got):
+            assert p == slots[b, :nbytes[b]].tobytes(), f"eps={eps} block {b}"
+        out = czw.decompress_field(rec, off, SHAPE, torch.float32, wtype, shuffle).cpu().numpy()
+        ref = oracle.decompress_field(slots, nbytes, SHAPE, wtype, "f32", shuffle)
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
