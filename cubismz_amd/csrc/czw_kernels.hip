@@ -1,7 +1,9 @@
 // czw_kernels.hip -- gfx950 kernels and C ABI (include/czw.h) of the wavelet
 // block compressor.
 //
-// One workgroup of 1024 threads (16 wave64) owns one 32^3 block.  The block
+// A persistent grid of 512-thread workgroups (8 wave64, one per CU slot the
+// LDS allows) walks the blocks; a workgroup owns one 32^3 block at a time
+// while the next block's input is prefetched into registers.  The block
 // stays in LDS in its ORIGINAL orientation O[z][y][x] for the whole
 // multilevel transform; each 1D pass loads one row per thread into
 // registers, transforms it there and stores it back.  The reference instead

@@ -1,0 +1,101 @@
+// czw_transform.hip -- whole-block multilevel transforms (forward, inverse and
+// the adjoint of the inverse) on blocks in the kernels' orientation
+// O[z][y][x], in binary32 or binary64, one launch per 1D pass.
+//
+// Used by the differentiable inverse (cubismz_amd.autograd): the inverse
+// transform with a frozen survivor mask is linear in the coefficients, and
+// its gradient is the adjoint of that linear map.  The pass sequence follows
+// the reference's schedule (WaveletsOnInterval.h:240-276 with the axis
+// permutation it implies):
+//   forward  level 32: x, y, z;  16: z, x, y;   8: y, z, x
+//   inverse  level  8: x, z, y;  16: y, x, z;  32: z, y, x
+//   adjoint of the inverse: the inverse passes in reverse order, each
+//   replaced by its transpose.
+// Coefficients stay in O orientation: the reference's stored ("physical")
+// coefficient array is the piecewise axis permutation of it documented in
+// czw_kernels.hip.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/czw.h"
+#include "czw_wavelet.hpp"
+
+namespace czw {
+
+enum { OP_FWD = 0, OP_INV = 1, OP_INV_ADJ = 2 };
+
+// Rows of length N along axis `ax` (0: z, 1: y, 2: x) inside the [0,N)^3
+// corner of every block; one thread per row.
+template <typename R, int WT, int OP, int N>
+__global__ __launch_bounds__(256) void k_rowpass(R *__restrict__ blocks, int nblocks, int ax)
+{
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (long long)nblocks * N * N) return;
+    const int b = (int)(r / (N * N)), i = (int)((r / N) % N), j = (int)(r % N);
+    const int stride = ax == 0 ? 1024 : (ax == 1 ? 32 : 1);
+    // the two other coordinates, slower first
+    const int si = ax == 0 ? 32 : 1024, sj = ax == 2 ? 32 : 1;
+    R *p = blocks + (size_t)b * 32768 + (size_t)i * si + (size_t)j * sj;
+    R v[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = p[k * stride];
+    if constexpr (OP == OP_FWD) fwd_row<N, WT, R>(v);
+    else if constexpr (OP == OP_INV) inv_row<N, WT, R>(v);
+    else adj_inv_row<N, WT, R>(v);
+#pragma unroll
+    for (int k = 0; k < N; ++k) p[k * stride] = v[k];
+}
+
+template <typename R, int WT, int OP>
+static void pass(R *blocks, int nblocks, int n, int ax, hipStream_t s)
+{
+    const long long rows = (long long)nblocks * n * n;
+    const dim3 grid((unsigned)((rows + 255) / 256)), blk(256);
+    if (n == 32) hipLaunchKernelGGL((k_rowpass<R, WT, OP, 32>), grid, blk, 0, s, blocks, nblocks, ax);
+    if (n == 16) hipLaunchKernelGGL((k_rowpass<R, WT, OP, 16>), grid, blk, 0, s, blocks, nblocks, ax);
+    if (n == 8) hipLaunchKernelGGL((k_rowpass<R, WT, OP, 8>), grid, blk, 0, s, blocks, nblocks, ax);
+}
+
+enum { AX_Z = 0, AX_Y = 1, AX_X = 2 };
+
+template <typename R, int WT>
+static void transform(R *blocks, int nblocks, int op, hipStream_t s)
+{
+    if (op == OP_FWD) {
+        const int sched[3][3] = {{AX_X, AX_Y, AX_Z}, {AX_Z, AX_X, AX_Y}, {AX_Y, AX_Z, AX_X}};
+        for (int l = 0; l < 3; ++l)
+            for (int k = 0; k < 3; ++k) pass<R, WT, OP_FWD>(blocks, nblocks, 32 >> l, sched[l][k], s);
+    } else {
+        const int sched[3][3] = {{AX_X, AX_Z, AX_Y}, {AX_Y, AX_X, AX_Z}, {AX_Z, AX_Y, AX_X}};  // levels 8, 16, 32
+        if (op == OP_INV) {
+            for (int l = 0; l < 3; ++l)
+                for (int k = 0; k < 3; ++k) pass<R, WT, OP_INV>(blocks, nblocks, 8 << l, sched[l][k], s);
+        } else {
+            for (int l = 2; l >= 0; --l)
+                for (int k = 2; k >= 0; --k) pass<R, WT, OP_INV_ADJ>(blocks, nblocks, 8 << l, sched[l][k], s);
+        }
+    }
+}
+
+template <typename R>
+static int transform_t(R *blocks, int nblocks, int wtype, int op, hipStream_t s)
+{
+    switch (wtype) {
+    case 1: transform<R, 1>(blocks, nblocks, op, s); break;
+    case 2: transform<R, 2>(blocks, nblocks, op, s); break;
+    case 0:
+    case 3: transform<R, 3>(blocks, nblocks, op, s); break;
+    default: return CZW_EINVAL;
+    }
+    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
+}
+
+}  // namespace czw
+
+extern "C" int czw_transform_blocks(void *blocks, int nblocks, int prec, int wtype, int op, hipStream_t stream)
+{
+    if (!blocks || nblocks <= 0 || op < 0 || op > 2) return CZW_EINVAL;
+    if (prec == 4) return czw::transform_t(static_cast<float *>(blocks), nblocks, wtype, op, stream);
+    if (prec == 8) return czw::transform_t(static_cast<double *>(blocks), nblocks, wtype, op, stream);
+    return CZW_EINVAL;
+}

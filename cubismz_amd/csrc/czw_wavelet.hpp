@@ -31,7 +31,7 @@ enum {
     K_0P0625 = 0, K_M0P0625, K_0P3125, K_M0P3125, K_0P5625, K_0P9375, K_1P3125, K_M2P1875, K_2P1875,
     K_0P125, K_M0P125, K_0P5, K_M0P5, K_0P625, K_1P375, K_COUNT
 };
-__constant__ double czw_kc[K_COUNT] = {0.0625, -0.0625, 0.3125, -0.3125, 0.5625, 0.9375, 1.3125, -2.1875, 2.1875,
+static __constant__ double czw_kc[K_COUNT] = {0.0625, -0.0625, 0.3125, -0.3125, 0.5625, 0.9375, 1.3125, -2.1875, 2.1875,
                                        0.125,  -0.125,  0.5,    -0.5,    0.625,  1.375};
 
 template <> struct Pred<float> {
@@ -181,6 +181,72 @@ CZW_DI void inv_row(R *d)
         }
         d[N - 2] = P::q0_last(s[H - 3], s[H - 2], s[H - 1]) - det[H - 1];
         d[N - 1] = P::q1_last(s[H - 3], s[H - 2], s[H - 1]) + det[H - 1];
+    }
+}
+
+// Adjoint (transpose) of one inverse level: given g = d(loss)/d(out) on the
+// N outputs of inv_row, returns d(loss)/d([s | det]) in the same [s | det]
+// layout.  The inverse is linear in exact arithmetic with the predictor
+// weights of WaveletsOnInterval.h:33-83; this applies the transposed weights
+// (transposed lifting).  Evaluated in R.
+template <int N, int WT, typename R>
+CZW_DI void adj_inv_row(R *g)
+{
+    constexpr int H = N / 2;
+    R gs[H], gd[H];
+#pragma unroll
+    for (int j = 0; j < H; ++j) gs[j] = 0;
+    if constexpr (WT == 1 || WT == 2) {
+        // out[2i] = s_i; out[2i+1] = P_i(s) + det_i
+#pragma unroll
+        for (int i = 0; i < H; ++i) {
+            gs[i] += g[2 * i];
+            gd[i] = g[2 * i + 1];
+        }
+        const R a = g[1];  // first: s0..s3
+        gs[0] += (R)0.3125 * a; gs[1] += (R)0.9375 * a; gs[2] += (R)-0.3125 * a; gs[3] += (R)0.0625 * a;
+#pragma unroll
+        for (int i = 1; i < H - 2; ++i) {  // middle: s_{i-1}..s_{i+2}
+            const R m = g[2 * i + 1];
+            gs[i - 1] += (R)-0.0625 * m; gs[i] += (R)0.5625 * m; gs[i + 1] += (R)0.5625 * m; gs[i + 2] += (R)-0.0625 * m;
+        }
+        const R o = g[N - 3], l = g[N - 1];  // one-to-last and last: s_{H-4}..s_{H-1}
+        gs[H - 4] += (R)0.0625 * o + (R)-0.3125 * l;
+        gs[H - 3] += (R)-0.3125 * o + (R)1.3125 * l;
+        gs[H - 2] += (R)0.9375 * o + (R)-2.1875 * l;
+        gs[H - 1] += (R)0.3125 * o + (R)2.1875 * l;
+        if constexpr (WT == 2) {  // the inverse first applies s -= det/2
+#pragma unroll
+            for (int i = 0; i < H; ++i) gd[i] -= (R)0.5 * gs[i];
+        }
+    } else {
+        // out[2i] = Q0_i(s) - det_i; out[2i+1] = Q1_i(s) + det_i
+#pragma unroll
+        for (int i = 0; i < H; ++i) gd[i] = g[2 * i + 1] - g[2 * i];
+        {
+            const R e = g[0], o = g[1];
+            gs[0] += (R)1.375 * e + (R)0.625 * o;
+            gs[1] += (R)-0.5 * e + (R)0.5 * o;
+            gs[2] += (R)0.125 * e + (R)-0.125 * o;
+        }
+#pragma unroll
+        for (int i = 1; i < H - 1; ++i) {
+            const R e = g[2 * i], o = g[2 * i + 1];
+            gs[i - 1] += (R)0.125 * e + (R)-0.125 * o;
+            gs[i] += e + o;
+            gs[i + 1] += (R)-0.125 * e + (R)0.125 * o;
+        }
+        {
+            const R e = g[N - 2], o = g[N - 1];
+            gs[H - 3] += (R)-0.125 * e + (R)0.125 * o;
+            gs[H - 2] += (R)0.5 * e + (R)-0.5 * o;
+            gs[H - 1] += (R)0.625 * e + (R)1.375 * o;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        g[i] = gs[i];
+        g[H + i] = gd[i];
     }
 }
 

@@ -65,6 +65,8 @@ def load_library():
     L.czw_decompress_field.argtypes = [vp, sz, vp, i, i, i, i, i, i, vp, vp, vp]
     L.czw_decompress_blocks.restype = i
     L.czw_decompress_blocks.argtypes = [vp, sz, vp, i, i, i, i, vp, vp, vp]
+    L.czw_transform_blocks.restype = i
+    L.czw_transform_blocks.argtypes = [vp, i, i, i, i, vp]
     _lib = L
     return L
 
@@ -236,3 +238,30 @@ def records_to_payloads(records, offsets):
         n = int(rec[off[b]:off[b] + 4].view(np.int32)[0])
         out.append(rec[off[b] + 4:off[b] + 4 + n].tobytes())
     return out
+
+
+TRANSFORM_OPS = {"fwt": 0, "iwt": 1, "iwt_adjoint": 2}
+
+
+def transform_blocks(blocks, wtype, op):
+    """In-place multilevel transform of blocks [nb,32,32,32] ([z][y][x]).
+
+    op "fwt": FullTransformEngine::fwt (FullWaveletTransform.h:44-49),
+    "iwt": iwt (:51-56), "iwt_adjoint": the transpose of the linear inverse.
+    Coefficients stay in the block's original orientation; see
+    :func:`cubismz_amd.autograd.to_physical` for the reference's layout.
+    """
+    torch = _torch()
+    if blocks.dim() != 4 or tuple(blocks.shape[1:]) != (BLOCK,) * 3 or not blocks.is_contiguous() or \
+            not blocks.is_cuda:
+        raise CZWError("blocks must be a contiguous device tensor [nb,32,32,32]")
+    if op not in TRANSFORM_OPS:
+        raise CZWError(f"unknown transform op {op!r}")
+    prec = _prec(blocks.dtype, torch)
+    nb = int(blocks.shape[0])
+    if nb == 0:
+        return blocks
+    rc = load_library().czw_transform_blocks(C.c_void_p(blocks.data_ptr()), nb, prec, int(wtype), TRANSFORM_OPS[op],
+                                             _stream(torch))
+    _check(rc, f"czw_transform_blocks({op})")
+    return blocks

@@ -117,6 +117,16 @@ int czw_decompress_blocks(const void *records, size_t records_bytes, const uint6
                           int nblocks, int prec, int wtype, int shuffle, void *blocks, uint32_t *err,
                           hipStream_t stream);
 
+/* Whole-block multilevel transform, in place, on nblocks blocks
+ * [nb][32][32][32] kept in their original orientation [z][y][x] (the
+ * coefficients are not permuted into the reference's stored layout).
+ * op 0: forward (FullTransformEngine::fwt, FullWaveletTransform.h:44-49),
+ * op 1: inverse (iwt, :51-56), op 2: adjoint (transpose) of the inverse, the
+ * backward of the differentiable inverse.  Binary32 ops 0/1 are bit-identical
+ * to the reference; op 2 is the exact transpose of the linear inverse.
+ * No counterpart call exists in the reference for op 2. */
+int czw_transform_blocks(void *blocks, int nblocks, int prec, int wtype, int op, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
