@@ -47,7 +47,122 @@ def parse():
     ap.add_argument("--bubbles", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full parity check")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-mode leg (64 uneven fields)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg (8 x 128^3)")
     return ap.parse_args()
+
+
+def _events(torch, n):
+    return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
+
+
+def batch_leg(a, torch, czw, fields):
+    """64 uneven fields, own seed and eps each: one persistent launch per stage
+    (graph-captured, autotuned launch shape) against 64 plain per-field calls."""
+    from cubismz_amd import batch as B
+
+    shapes, eps = fields.batched_shapes(64, seed=7)
+    fs = [fields.bubble_field(s, seed=100 + k, nbubbles=24, dtype=torch.float32, device="cuda")
+          for k, s in enumerate(shapes)]
+    total_bytes = sum(f.numel() * 4 for f in fs)
+    bt = B.Batch(fs, list(eps), wtype=a.wtype)
+    # plain: one compress_field + one decompress_field per field, buffers preallocated
+    bufs = []
+    for f in fs:
+        nb = f.numel() // 32768
+        bufs.append((torch.empty(czw.records_bound(nb, 4), dtype=torch.uint8, device="cuda"),
+                     torch.empty(nb + 1, dtype=torch.int64, device="cuda"),
+                     torch.empty(czw.workspace_bytes(nb, 4), dtype=torch.uint8, device="cuda"),
+                     torch.empty_like(f)))
+    err = torch.zeros(1, dtype=torch.int32, device="cuda")
+
+    def plain():
+        for k, f in enumerate(fs):
+            rec, off, ws, out = bufs[k]
+            czw.compress_field(f, float(eps[k]), a.wtype, 0, False, rec, off, ws)
+            czw.decompress_field(rec, off, f.shape, torch.float32, a.wtype, False, out=out, err=err, check=False)
+
+    reps = max(3, a.steps // 4)
+    for _ in range(2):
+        plain()
+    e = _events(torch, 2)
+    e[0].record()
+    for _ in range(reps):
+        plain()
+    e[1].record()
+    torch.cuda.synchronize()
+    t_plain = e[0].elapsed_time(e[1]) / reps
+    ms_c = bt.autotune("compress")
+    ms_d = bt.autotune("decompress")
+    shape = bt.launch_shape
+    bt.compress()
+    bt.decompress(check=True)
+    g = bt.graph(("compress", "decompress"))
+    for _ in range(2):
+        g.launch()
+    e[0].record()
+    for _ in range(reps):
+        g.launch()
+    e[1].record()
+    torch.cuda.synchronize()
+    t_graph = e[0].elapsed_time(e[1]) / reps
+    g.close()
+    same = True
+    for k in (0, 31, 63):
+        rec, off, _, out = bufs[k]
+        brec, boff = bt.field_records(k)
+        same &= bool(torch.equal(boff, off)) and bool(torch.equal(brec, rec[: int(off[-1])]))
+        same &= bool(torch.equal(bt.outputs[k].view(torch.int32), out.view(torch.int32)))
+    return {"fields": 64, "blocks": bt.nblocks, "field_bytes": total_bytes,
+            "GBps_batched": round(total_bytes / (t_graph * 1e-3) / 1e9, 3),
+            "GBps_plain_64_launches": round(total_bytes / (t_plain * 1e-3) / 1e9, 3),
+            "ms_batched": round(t_graph, 4), "ms_plain": round(t_plain, 4),
+            "speedup_vs_plain": round(t_plain / t_graph, 3),
+            "autotuned": {"compress_ms": round(ms_c, 4), "decompress_ms": round(ms_d, 4), **shape},
+            "matches_per_field": same}
+
+
+def adjoint_leg(a, torch, fields):
+    """8 x 128^3 f32: masked inverse (forward) and its adjoint (backward)."""
+    from cubismz_amd import autograd as AG
+
+    blocks = torch.cat([AG.field_to_blocks(fields.bumps_field((128, 128, 128), seed=k, dtype=torch.float32,
+                                                              device="cuda")) for k in range(8)])
+    co = AG.fwt(blocks, a.wtype)
+    m = AG.threshold_mask(co, 1e-3)
+    c = co.clone()
+    g = torch.randn_like(co)
+    nbytes = co.numel() * 4
+    from cubismz_amd import czw
+
+    reps = max(5, a.steps)
+    out = {}
+    for op, x in (("iwt", c), ("iwt_adjoint", g)):
+        buf = x.clone()
+        czw.transform_blocks(buf, a.wtype, op)
+        e = _events(torch, 2)
+        e[0].record()
+        for _ in range(reps):
+            czw.transform_blocks(buf, a.wtype, op)
+        e[1].record()
+        torch.cuda.synchronize()
+        t = e[0].elapsed_time(e[1]) / reps
+        out[op] = {"ms": round(t, 4), "GBps": round(nbytes / (t * 1e-3) / 1e9, 3)}
+    cc = c.clone().requires_grad_(True)
+    y = AG.masked_iwt(cc, m, a.wtype)
+    y.backward(g)
+    torch.cuda.synchronize()
+    e = _events(torch, 2)
+    e[0].record()
+    for _ in range(reps):
+        cc.grad = None
+        AG.masked_iwt(cc, m, a.wtype).backward(g)
+    e[1].record()
+    torch.cuda.synchronize()
+    t = e[0].elapsed_time(e[1]) / reps
+    out["autograd_fwd_bwd"] = {"ms": round(t, 4), "GBps": round(nbytes / (t * 1e-3) / 1e9, 3)}
+    out["blocks"] = int(blocks.shape[0])
+    return out
 
 
 def main():
@@ -136,6 +251,8 @@ def main():
         torch.cuda.synchronize()
         tw = e0.elapsed_time(e1) / a.steps * 1e-3
         other[f"wtype{w}"] = {"GBps": round(field_bytes / tw / 1e9, 3), "ms_per_step": round(tw * 1e3, 4)}
+    batched = None if a.no_batch else batch_leg(a, torch, czw, fields)
+    adjoint = None if a.no_adjoint else adjoint_leg(a, torch, fields)
     # restore the headline wavelet's records for the checks below
     step()
     torch.cuda.synchronize()
@@ -208,6 +325,8 @@ def main():
                      "alg_bytes_per_launch": kern[dom][0], "avg_launch_ms": round(kern[dom][1] * 1e3, 4)},
         "step_roofline_frac": round((alg_c + alg_d) / t_step / HBM_PEAK, 4),
         "other_wtypes": other,
+        "batched": batched,
+        "adjoint": adjoint,
         "compression": {"payload_bytes": payload, "survivor_fraction": round(rho, 5),
                         "ratio_vs_records": round(field_bytes / float(off[-1]), 3),
                         "linf": err_inf, "psnr_db": round(float(psnr), 4)},

@@ -43,7 +43,8 @@ constexpr int NT = 512;            // threads per block workgroup (8 wave64)
 constexpr int NROW = 1024 / NT;    // level-32 rows (and stream segments) per thread
 constexpr size_t SLOT32 = MASK_BYTES + 4 * (size_t)NB3;
 
-enum { LAYOUT_BLOCKS = 0, LAYOUT_FIELD = 1 };
+enum { LAYOUT_BLOCKS = 0, LAYOUT_FIELD = 1, LAYOUT_BATCH = 2 };
+constexpr int BATCH_MAX_FIELDS = 128;  // field table entries one launch keeps in LDS
 
 template <int N, typename R>
 CZW_DI void lds_ld(const R *A, int base, int stride, R *r)
@@ -269,6 +270,56 @@ struct FieldGeom {
     int nx, ny, nz;
 };
 
+// The block a workgroup works on, resolved from the batch-wide index b: the
+// field it belongs to (source or destination, geometry, threshold) and its
+// index lb inside that field.  Outside LAYOUT_BATCH this is the launch's one
+// field and lb = b.
+struct BlockRef {
+    const float *src;
+    float *dst;
+    FieldGeom g;
+    int lb;
+    float eps;
+};
+
+// Field table of a batch launch, copied into LDS once per workgroup.
+template <int LAYOUT>
+CZW_DI void load_table(czw_field_t *T, const czw_field_t *__restrict__ fields, int nf)
+{
+    if constexpr (LAYOUT == LAYOUT_BATCH) {
+        constexpr int W = sizeof(czw_field_t) / 4;
+        const uint32_t *s = reinterpret_cast<const uint32_t *>(fields);
+        uint32_t *d = reinterpret_cast<uint32_t *>(T);
+        for (int i = threadIdx.x; i < nf * W; i += blockDim.x) d[i] = s[i];
+        __syncthreads();
+    }
+}
+
+template <int LAYOUT>
+CZW_DI BlockRef locate(const czw_field_t *T, int nf, int b, const float *src, float *dst, FieldGeom g, float eps)
+{
+    if constexpr (LAYOUT != LAYOUT_BATCH) {
+        return BlockRef{src, dst, g, b, eps};
+    } else {
+        int lo = 0, hi = nf - 1;  // the last field whose first block is <= b
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (T[mid].block0 <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        const czw_field_t &f = T[lo];
+        return BlockRef{static_cast<const float *>(f.src), static_cast<float *>(f.dst), FieldGeom{f.nx, f.ny, f.nz},
+                        b - f.block0, f.eps};
+    }
+}
+
+// Load/store layout of a block's rows: batch fields are HDF5-ordered fields.
+template <int LAYOUT>
+constexpr int row_layout()
+{
+    return LAYOUT == LAYOUT_BLOCKS ? LAYOUT_BLOCKS : LAYOUT_FIELD;
+}
+
 // Global element offset of block b's row start (x = 0) for the x-row (y, z)
 // of an HDF5-ordered field, and the stride along x.
 CZW_DI size_t field_row(const FieldGeom &g, int b, int y, int z, size_t *xstride)
@@ -387,33 +438,48 @@ CZW_DI void corner_piece(int p, F &&f)
     }
 }
 
+//
+// Block order: workgroup w starts with block w; the next block is w + G
+// (static) or, when `queue` is given, the next ticket G + atomicAdd(queue, 1)
+// of a work queue whose counter the caller has cleared.  In LAYOUT_BATCH the
+// blocks of nf fields (table `fields`, batch-wide numbering) form one queue.
 template <int WT, int LAYOUT>
 __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ src, FieldGeom g, int nblocks,
                                                      float eps, uint32_t zkeep, uint8_t *__restrict__ slots,
-                                                     uint32_t *__restrict__ nbytes)
+                                                     uint32_t *__restrict__ nbytes,
+                                                     const czw_field_t *__restrict__ fields, int nf,
+                                                     uint32_t *__restrict__ queue)
 {
+    constexpr int RL = row_layout<LAYOUT>();
     __shared__ __align__(16) float A[LDS_WORDS];
     __shared__ uint32_t M[1024];
     __shared__ uint32_t cnt[2048];  // level-32 run counts, then their exclusive offsets
     __shared__ uint32_t wsum[NT / 64];
     __shared__ uint32_t s32_total;
+    __shared__ int ticket[2];
+    __shared__ czw_field_t T[LAYOUT == LAYOUT_BATCH ? BATCH_MAX_FIELDS : 1];
     const int t = threadIdx.x;
     const int G = gridDim.x;
+    load_table<LAYOUT>(T, fields, nf);
     float nx[NROW][32];
     int b = blockIdx.x;
+    float eps_b = eps;
     if (b < nblocks) {
+        const BlockRef br = locate<LAYOUT>(T, nf, b, src, nullptr, g, eps);
+        eps_b = br.eps;
 #pragma unroll
-        for (int k = 0; k < NROW; ++k) load_xrow_f32<LAYOUT>(src, g, b, t + NT * k, nx[k]);
+        for (int k = 0; k < NROW; ++k) load_xrow_f32<RL>(br.src, br.g, br.lb, t + NT * k, nx[k]);
     }
-    for (; b < nblocks; b += G) {
+    for (int it = 0; b < nblocks; ++it) {
         uint8_t *slot = slots + (size_t)b * SLOT32;
+        if (queue && t == 0) ticket[it & 1] = G + (int)atomicAdd(queue, 1u);
 #pragma unroll
         for (int k = 0; k < NROW; ++k) {  // level 32, x-pass
             float r[32];
 #pragma unroll
             for (int i = 0; i < 32; ++i) r[i] = nx[k][i];
             row<32, true, WT>(r);
-            lds_st<32>(A, xrow_lds_base<LAYOUT>(t + NT * k), 1, r);
+            lds_st<32>(A, xrow_lds_base<RL>(t + NT * k), 1, r);
         }
         if (t < 2048 - NRUN32) cnt[NRUN32 + t] = 0u;
         __syncthreads();
@@ -421,8 +487,14 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
         // 63 outstanding loads a wave may have (more would stall the wave
         // until the first returns).  The last block reloads itself, so nx is
         // always written and is dead between its use and its reload.
-        const int bn = min(b + G, nblocks - 1);
-        load_xrow_f32<LAYOUT>(src, g, bn, t, nx[0]);
+        const int bnext = queue ? ticket[it & 1] : b + G;
+        const int bn = min(bnext, nblocks - 1);
+        const float eps = eps_b;
+        {
+            const BlockRef br = locate<LAYOUT>(T, nf, bn, src, nullptr, g, eps);
+            eps_b = br.eps;
+            load_xrow_f32<RL>(br.src, br.g, br.lb, t, nx[0]);
+        }
 #pragma unroll
         for (int k = 0; k < NROW; ++k) {  // level 32, y-pass: row (z, x)
             const int q = t + NT * k, z = q >> 5, x = q & 31;
@@ -517,7 +589,10 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
                 }
             }
         }
-        load_xrow_f32<LAYOUT>(src, g, bn, t + NT, nx[1]);  // second half of the prefetch
+        {  // second half of the prefetch
+            const BlockRef br = locate<LAYOUT>(T, nf, bn, src, nullptr, g, eps);
+            load_xrow_f32<RL>(br.src, br.g, br.lb, t + NT, nx[1]);
+        }
         __syncthreads();  // stage complete, wsum free again
         if (staged) {
             const uint4 *s4 = reinterpret_cast<const uint4 *>(st32);
@@ -560,6 +635,7 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
             nbytes[nblocks + b] = Sc;
         }
         __syncthreads();  // A, M, cnt and wsum are read before the next block reuses them
+        b = bnext;
     }
 }
 
@@ -804,30 +880,48 @@ CZW_DI bool decode_block_fast_f32(float *A, uint32_t *M, uint32_t *cnt, uint32_t
     return true;
 }
 
+//
+// Block order as in the compressor, three deep: the record of the next block
+// b1 and the offset of the one after, b2, are in flight while b is decoded.
+// Workgroup w starts with w, w + G, w + 2G; then b2 advances by G (static) or
+// takes the ticket 3G + atomicAdd(queue, 1) of a cleared work queue.
 template <int WT, int LAYOUT>
 __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict__ records, uint64_t rec_bytes,
                                                        const uint64_t *__restrict__ offsets, int nblocks,
                                                        int shuffle, float *__restrict__ dst, FieldGeom g,
-                                                       uint32_t *__restrict__ err)
+                                                       uint32_t *__restrict__ err,
+                                                       const czw_field_t *__restrict__ fields, int nf,
+                                                       uint32_t *__restrict__ queue)
 {
+    constexpr int RL = row_layout<LAYOUT>();
     __shared__ __align__(16) float A[LDS_WORDS];
     __shared__ uint32_t M[1024];
     __shared__ uint32_t cnt[2560];
     __shared__ uint32_t wsum[NT / 64];
     __shared__ uint32_t flag;
+    __shared__ int ticket[2];
+    __shared__ czw_field_t T[LAYOUT == LAYOUT_BATCH ? BATCH_MAX_FIELDS : 1];
     const int t = threadIdx.x;
     const int G = gridDim.x;
-    int b = blockIdx.x;
+    load_table<LAYOUT>(T, fields, nf);
+    int b = blockIdx.x, b1 = b + G, b2 = b + 2 * G, bq = 0;
     uint64_t ob = b < nblocks ? offsets[b] : 0;
-    uint64_t ob_next = b + G < nblocks ? offsets[b + G] : 0;
+    uint64_t ob_next = b1 < nblocks ? offsets[b1] : 0;
     RecPrefetch pf;
     if (b < nblocks) fetch_record(records, rec_bytes, ob, t, pf);
-    for (; b < nblocks; b += G) {
+    for (int it = 0; b < nblocks; ++it, b = b1, b1 = b2, b2 = bq) {
+        if (queue) {
+            if (t == 0) ticket[it & 1] = 3 * G + (int)atomicAdd(queue, 1u);
+            __syncthreads();
+            bq = ticket[it & 1];
+        } else {
+            bq = b2 + G;
+        }
         const RecPrefetch cur = pf;
         const uint64_t ob_cur = ob;
         ob = ob_next;
-        if (b + 2 * G < nblocks) ob_next = offsets[b + 2 * G];
-        if (b + G < nblocks) fetch_record(records, rec_bytes, ob, t, pf);
+        if (b2 < nblocks) ob_next = offsets[b2];
+        if (b1 < nblocks) fetch_record(records, rec_bytes, ob, t, pf);
 
         const uint32_t nb = cur.nb;
         if (nb < MASK_BYTES || ((nb - MASK_BYTES) & 3u) || nb > SLOT32 || ob_cur + 4 + nb > rec_bytes) {
@@ -931,13 +1025,13 @@ __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict
             }
             __syncthreads();
         }
-        if constexpr (LAYOUT == LAYOUT_BLOCKS) {
+        if constexpr (RL == LAYOUT_BLOCKS) {
 #pragma unroll
             for (int k = 0; k < NROW; ++k) {
                 float rr[32];
-                lds_ld<32>(A, xrow_lds_base<LAYOUT>(t + NT * k), 1, rr);
+                lds_ld<32>(A, xrow_lds_base<RL>(t + NT * k), 1, rr);
                 row<32, false, WT>(rr);  // level 32, x-pass
-                store_xrow_f32<LAYOUT>(dst, g, b, t + NT * k, rr);
+                store_xrow_f32<RL>(dst, g, b, t + NT * k, rr);
             }
         } else {
             // level 32 x-pass back into A, then 16-byte stores along the
@@ -952,14 +1046,16 @@ __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict
                 lds_st<32>(A, base, 1, rr);
             }
             __syncthreads();
+            const BlockRef br = locate<LAYOUT>(T, nf, b, nullptr, dst, g, 0.0f);
             size_t xs;
-            float *p0 = dst + field_row(g, b, 0, 0, &xs);
+            float *p0 = br.dst + field_row(br.g, br.lb, 0, 0, &xs);
+            const int nz = br.g.nz;
 #pragma unroll 4
             for (int i = t; i < 32 * 32 * 8; i += NT) {  // (x, y, z4) with z4 fastest
                 const int z4 = i & 7, y = (i >> 3) & 31, x = i >> 8;
                 const int a = (4 * z4) * SP + y * RP + x;
                 const float4 v = make_float4(A[a], A[a + SP], A[a + 2 * SP], A[a + 3 * SP]);
-                *reinterpret_cast<float4 *>(p0 + x * xs + (size_t)y * g.nz + 4 * z4) = v;
+                *reinterpret_cast<float4 *>(p0 + x * xs + (size_t)y * nz + 4 * z4) = v;
             }
         }
     }
@@ -1008,9 +1104,11 @@ static int compress_slots_f32(const float *src, int nblocks, FieldGeom g, float 
     const void *fn = wt == 1 ? (const void *)k_compress_f32<1, LAYOUT>
                    : wt == 2 ? (const void *)k_compress_f32<2, LAYOUT> : (const void *)k_compress_f32<3, LAYOUT>;
     const dim3 grid(std::min(nblocks, resident_groups(fn, NT))), blk(NT);
-    if (wt == 1) hipLaunchKernelGGL((k_compress_f32<1, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb);
-    if (wt == 2) hipLaunchKernelGGL((k_compress_f32<2, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb);
-    if (wt == 3) hipLaunchKernelGGL((k_compress_f32<3, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb);
+    const czw_field_t *nf0 = nullptr;
+    uint32_t *nq = nullptr;
+    if (wt == 1) hipLaunchKernelGGL((k_compress_f32<1, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, nf0, 0, nq);
+    if (wt == 2) hipLaunchKernelGGL((k_compress_f32<2, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, nf0, 0, nq);
+    if (wt == 3) hipLaunchKernelGGL((k_compress_f32<3, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, nf0, 0, nq);
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
 
@@ -1046,9 +1144,11 @@ static int decompress_f32(const void *records, size_t rec_bytes, const uint64_t 
     const void *fn = wt == 1 ? (const void *)k_decompress_f32<1, LAYOUT>
                    : wt == 2 ? (const void *)k_decompress_f32<2, LAYOUT> : (const void *)k_decompress_f32<3, LAYOUT>;
     const dim3 grid(std::min(nblocks, resident_groups(fn, NT))), blk(NT);
-    if (wt == 1) hipLaunchKernelGGL((k_decompress_f32<1, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, nblocks, shuffle, dst, g, err);
-    if (wt == 2) hipLaunchKernelGGL((k_decompress_f32<2, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, nblocks, shuffle, dst, g, err);
-    if (wt == 3) hipLaunchKernelGGL((k_decompress_f32<3, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, nblocks, shuffle, dst, g, err);
+    const czw_field_t *nf0 = nullptr;
+    uint32_t *nq = nullptr;
+    if (wt == 1) hipLaunchKernelGGL((k_decompress_f32<1, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, nblocks, shuffle, dst, g, err, nf0, 0, nq);
+    if (wt == 2) hipLaunchKernelGGL((k_decompress_f32<2, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, nblocks, shuffle, dst, g, err, nf0, 0, nq);
+    if (wt == 3) hipLaunchKernelGGL((k_decompress_f32<3, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, nblocks, shuffle, dst, g, err, nf0, 0, nq);
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
 
@@ -1056,6 +1156,75 @@ static bool field_ok(int nx, int ny, int nz)
 {
     return nx > 0 && ny > 0 && nz > 0 && !(nx & 31) && !(ny & 31) && !(nz & 31) &&
            (size_t)(nx / 32) * (ny / 32) * (nz / 32) < (size_t)INT32_MAX;
+}
+
+// ---------------------------------------------------------------------------
+// batched mode
+
+static_assert(sizeof(czw_field_t) == 40, "czw_field_t layout");
+static_assert(BATCH_MAX_FIELDS == CZW_BATCH_MAX_FIELDS, "field table size");
+
+// Clears n words; used instead of a memset so that captured graphs replay it.
+__global__ void k_fill_u32(uint32_t *__restrict__ p, int n, uint32_t v)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+static uint32_t *batch_queue(const czw_batch_t *B)
+{
+    const size_t slot = MASK_BYTES + (size_t)B->prec * NB3;
+    return reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(B->workspace) + (slot + 8) * (size_t)B->nblocks);
+}
+
+static bool batch_ok(const czw_batch_t *B)
+{
+    return B && B->fields && B->nfields > 0 && B->nfields <= BATCH_MAX_FIELDS && B->nblocks > 0 && B->workspace &&
+           B->records && B->offsets && B->groups >= 0 && wt_index(B->wtype) > 0;
+}
+
+static int batch_compress(const czw_batch_t *B, hipStream_t s)
+{
+    if (!batch_ok(B) || !zb_ok(B->zerobits)) return CZW_EINVAL;
+    if (B->prec != 4) return B->prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
+    const int wt = wt_index(B->wtype);
+    uint8_t *slots = static_cast<uint8_t *>(B->workspace);
+    uint32_t *nb = reinterpret_cast<uint32_t *>(slots + SLOT32 * (size_t)B->nblocks);
+    uint32_t *q = B->queue ? batch_queue(B) : nullptr;
+    if (q) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, q, 1, 0u);
+    const uint32_t zkeep = B->zerobits ? ~((1u << B->zerobits) - 1u) : 0xffffffffu;
+    const void *fn = wt == 1 ? (const void *)k_compress_f32<1, LAYOUT_BATCH>
+                   : wt == 2 ? (const void *)k_compress_f32<2, LAYOUT_BATCH> : (const void *)k_compress_f32<3, LAYOUT_BATCH>;
+    const int groups = B->groups ? B->groups : resident_groups(fn, NT);
+    const dim3 grid(std::min(B->nblocks, groups)), blk(NT);
+    const float *src0 = nullptr;
+    const FieldGeom g{0, 0, 0};
+    if (wt == 1) hipLaunchKernelGGL((k_compress_f32<1, LAYOUT_BATCH>), grid, blk, 0, s, src0, g, B->nblocks, 0.0f, zkeep, slots, nb, B->fields, B->nfields, q);
+    if (wt == 2) hipLaunchKernelGGL((k_compress_f32<2, LAYOUT_BATCH>), grid, blk, 0, s, src0, g, B->nblocks, 0.0f, zkeep, slots, nb, B->fields, B->nfields, q);
+    if (wt == 3) hipLaunchKernelGGL((k_compress_f32<3, LAYOUT_BATCH>), grid, blk, 0, s, src0, g, B->nblocks, 0.0f, zkeep, slots, nb, B->fields, B->nfields, q);
+    if (hipGetLastError() != hipSuccess) return CZW_EHIP;
+    return pack_records(slots, SLOT32, nb, B->nblocks, B->shuffle, static_cast<uint8_t *>(B->records), B->offsets, s);
+}
+
+static int batch_decompress(const czw_batch_t *B, hipStream_t s)
+{
+    if (!batch_ok(B) || !B->err) return CZW_EINVAL;
+    if (B->prec != 4) return B->prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
+    const int wt = wt_index(B->wtype);
+    uint32_t *q = B->queue ? batch_queue(B) + 1 : nullptr;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, B->err, 1, 0u);
+    if (q) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, q, 1, 0u);
+    const uint8_t *rec = static_cast<const uint8_t *>(B->records);
+    const void *fn = wt == 1 ? (const void *)k_decompress_f32<1, LAYOUT_BATCH>
+                   : wt == 2 ? (const void *)k_decompress_f32<2, LAYOUT_BATCH> : (const void *)k_decompress_f32<3, LAYOUT_BATCH>;
+    const int groups = B->groups ? B->groups : resident_groups(fn, NT);
+    const dim3 grid(std::min(B->nblocks, groups)), blk(NT);
+    float *dst0 = nullptr;
+    const FieldGeom g{0, 0, 0};
+    const uint64_t rb = (uint64_t)B->records_bytes;
+    if (wt == 1) hipLaunchKernelGGL((k_decompress_f32<1, LAYOUT_BATCH>), grid, blk, 0, s, rec, rb, B->offsets, B->nblocks, B->shuffle, dst0, g, B->err, B->fields, B->nfields, q);
+    if (wt == 2) hipLaunchKernelGGL((k_decompress_f32<2, LAYOUT_BATCH>), grid, blk, 0, s, rec, rb, B->offsets, B->nblocks, B->shuffle, dst0, g, B->err, B->fields, B->nfields, q);
+    if (wt == 3) hipLaunchKernelGGL((k_decompress_f32<3, LAYOUT_BATCH>), grid, blk, 0, s, rec, rb, B->offsets, B->nblocks, B->shuffle, dst0, g, B->err, B->fields, B->nfields, q);
+    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
 
 } // namespace czw
@@ -1142,6 +1311,99 @@ int czw_decompress_blocks(const void *records, size_t records_bytes, const uint6
         return decompress_f32<LAYOUT_BLOCKS>(records, records_bytes, offsets, nblocks, g, wtype, shuffle,
                                              static_cast<float *>(blocks), err, stream);
     return prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
+}
+
+int czw_batch_prepare(czw_field_t *fields, int nfields, int *nblocks)
+{
+    if (!fields || !nblocks || nfields <= 0 || nfields > BATCH_MAX_FIELDS) return CZW_EINVAL;
+    int64_t total = 0;
+    for (int f = 0; f < nfields; ++f) {
+        czw_field_t &d = fields[f];
+        if (!field_ok(d.nx, d.ny, d.nz) || !(d.eps >= 0.0f)) return CZW_EINVAL;
+        d.block0 = (int32_t)total;
+        d.nblocks = (d.nx / 32) * (d.ny / 32) * (d.nz / 32);
+        total += d.nblocks;
+        if (total >= INT32_MAX / 4) return CZW_EINVAL;
+    }
+    *nblocks = (int)total;
+    return CZW_OK;
+}
+
+size_t czw_batch_workspace_bytes(int nblocks, int prec)
+{
+    const size_t w = czw_compress_workspace_bytes(nblocks, prec);
+    return w ? w + 256 : 0;
+}
+
+int czw_batch_compress(const czw_batch_t *batch, hipStream_t stream) { return batch_compress(batch, stream); }
+
+int czw_batch_decompress(const czw_batch_t *batch, hipStream_t stream) { return batch_decompress(batch, stream); }
+
+int czw_batch_autotune(czw_batch_t *batch, int stage, hipStream_t stream, float *best_ms)
+{
+    if (!batch_ok(batch) || (stage != 1 && stage != 2)) return CZW_EINVAL;
+    const void *fn = (const void *)k_compress_f32<1, LAYOUT_BATCH>;
+    const int R = resident_groups(fn, NT);
+    const int shapes[4][2] = {{R, 0}, {R, 1}, {R / 2 > 0 ? R / 2 : 1, 0}, {R / 2 > 0 ? R / 2 : 1, 1}};
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return CZW_EHIP;
+    czw_batch_t trial = *batch;
+    float best = 1e30f;
+    int bg = R, bq = 0, rc = CZW_OK;
+    for (int k = 0; k < 4 && rc == CZW_OK; ++k) {
+        trial.groups = shapes[k][0];
+        trial.queue = shapes[k][1];
+        auto run = [&]() { return stage == 1 ? batch_compress(&trial, stream) : batch_decompress(&trial, stream); };
+        if ((rc = run()) != CZW_OK) break;  // warm-up
+        (void)hipEventRecord(e0, stream);
+        for (int r = 0; r < 3 && rc == CZW_OK; ++r) rc = run();
+        (void)hipEventRecord(e1, stream);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = CZW_EHIP;
+        float ms = 0.0f;
+        if (rc == CZW_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms / 3 < best) {
+            best = ms / 3;
+            bg = trial.groups;
+            bq = trial.queue;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != CZW_OK) return rc;
+    batch->groups = bg;
+    batch->queue = bq;
+    if (best_ms) *best_ms = best;
+    return CZW_OK;
+}
+
+int czw_batch_graph(const czw_batch_t *batch, int stages, void **graph)
+{
+    if (!batch_ok(batch) || !graph || stages < 1 || stages > 3) return CZW_EINVAL;
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return CZW_EHIP;
+    int rc = CZW_OK;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed) != hipSuccess) rc = CZW_EHIP;
+    if (rc == CZW_OK && (stages & 1)) rc = batch_compress(batch, cs);
+    if (rc == CZW_OK && (stages & 2)) rc = batch_decompress(batch, cs);
+    // end the capture whatever happened, so the stream is usable again
+    if (hipStreamEndCapture(cs, &g) != hipSuccess && rc == CZW_OK) rc = CZW_EHIP;
+    if (rc == CZW_OK && hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) != hipSuccess) rc = CZW_EHIP;
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(cs);
+    if (rc == CZW_OK) *graph = ex;
+    return rc;
+}
+
+int czw_graph_launch(void *graph, hipStream_t stream)
+{
+    if (!graph) return CZW_EINVAL;
+    return hipGraphLaunch(static_cast<hipGraphExec_t>(graph), stream) == hipSuccess ? CZW_OK : CZW_EHIP;
+}
+
+void czw_graph_destroy(void *graph)
+{
+    if (graph) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph));
 }
 
 } // extern "C"

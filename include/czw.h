@@ -117,6 +117,66 @@ int czw_decompress_blocks(const void *records, size_t records_bytes, const uint6
                           int nblocks, int prec, int wtype, int shuffle, void *blocks, uint32_t *err,
                           hipStream_t stream);
 
+/* ---------------------------------------------------------------------
+ * Batched mode: many fields of different shapes and thresholds in one
+ * persistent launch per stage.  No counterpart call exists in the reference,
+ * whose writer compresses one field per process (hdf2cz); a batch is the
+ * concatenation of what per-field calls would produce.
+ *
+ * Blocks are numbered batch-wide: field f owns [block0, block0 + nblocks),
+ * each field in its own block order (see "Field layout").  The records of a
+ * batch are the per-field record streams back to back, so field f's record
+ * stream is records[offsets[block0] .. offsets[block0 + nblocks]) with
+ * offsets rebased by offsets[block0].
+ */
+typedef struct czw_field {
+    const void *src; /* compress: the field, HDF5 order [nx][ny][nz], device */
+    void *dst;       /* decompress: output field, same layout, device */
+    int32_t nx, ny, nz; /* multiples of 32 */
+    float eps;       /* this field's threshold */
+    int32_t block0;  /* set by czw_batch_prepare */
+    int32_t nblocks; /* set by czw_batch_prepare */
+} czw_field_t;
+
+#define CZW_BATCH_MAX_FIELDS 128
+
+typedef struct czw_batch {
+    const czw_field_t *fields; /* DEVICE copy of the prepared table */
+    int nfields;               /* 1 .. CZW_BATCH_MAX_FIELDS */
+    int nblocks;               /* total, from czw_batch_prepare */
+    int prec, wtype, zerobits, shuffle;
+    void *workspace;           /* czw_batch_workspace_bytes(nblocks, prec) */
+    void *records;             /* czw_records_bound(nblocks, prec) bytes */
+    uint64_t *offsets;         /* nblocks + 1 */
+    size_t records_bytes;      /* size of the records buffer (decompress bound) */
+    uint32_t *err;             /* decompress error bits; cleared by each decompress */
+    int groups;                /* persistent workgroups; 0 = as many as are resident */
+    int queue;                 /* 1: dynamic block queue; 0: static stride */
+} czw_batch_t;
+
+/* Host: validate a table, fill block0/nblocks, return the total in *nblocks. */
+int czw_batch_prepare(czw_field_t *fields, int nfields, int *nblocks);
+
+/* Scratch of a batch: compressor slots and sizes, then 256 B of queue counters. */
+size_t czw_batch_workspace_bytes(int nblocks, int prec);
+
+/* Compress every field (fwt, thresholds, survivor streams, records). */
+int czw_batch_compress(const czw_batch_t *batch, hipStream_t stream);
+
+/* Decompress every field from batch->records into each field's dst. */
+int czw_batch_decompress(const czw_batch_t *batch, hipStream_t stream);
+
+/* Time the launch shapes (groups x queue) of one stage (1 compress,
+ * 2 decompress) on this batch and store the fastest in batch->groups/queue;
+ * *best_ms receives its time.  Runs the stage several times on `stream`. */
+int czw_batch_autotune(czw_batch_t *batch, int stage, hipStream_t stream, float *best_ms);
+
+/* Capture the stages (bit 0 compress, bit 1 decompress) of a batch into an
+ * executable HIP graph; launch it with czw_graph_launch. */
+int czw_batch_graph(const czw_batch_t *batch, int stages, void **graph);
+int czw_graph_launch(void *graph, hipStream_t stream);
+void czw_graph_destroy(void *graph);
+
 /* Whole-block multilevel transform, in place, on nblocks blocks
  * [nb][32][32][32] kept in their original orientation [z][y][x] (the
  * coefficients are not permuted into the reference's stored layout).

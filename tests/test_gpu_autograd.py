@@ -141,7 +141,7 @@ def test_masked_iwt_f32_gradient_matches_f64(torch_gpu):
     m = A.threshold_mask(co, 1e-3)
     grads = []
     for dt in (torch.float32, torch.float64):
-        c = co.to(dt).requires_grad_(True)
+        c = co.detach().to(dt).clone().requires_grad_(True)
         ((A.masked_iwt(c, m, wtype) - t.to(dt)) ** 2).sum().backward()
         grads.append(c.grad.double())
     assert float((grads[0] - grads[1]).abs().max()) <= 1e-5 * float(grads[1].abs().max())

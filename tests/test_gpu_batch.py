@@ -1,0 +1,121 @@
+"""GPU tests of batched mode: one persistent launch per stage over the blocks
+of many uneven fields, each with its own eps.  Every field's records and
+reconstruction must equal, byte for byte, what the per-field entry points
+give for it alone (those are pinned against the oracle elsewhere; one field
+is re-checked against the oracle here)."""
+import numpy as np
+import pytest
+
+from cubismz_amd import batch as B
+from cubismz_amd import czw, fields
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(64, 32, 96), (32, 32, 32), (96, 64, 32), (32, 96, 64), (64, 64, 64), (32, 32, 128), (128, 32, 32)]
+EPS = [1e-3, 1e-4, 5e-3, 2e-4, 1e-2, 3e-3, 7e-4]
+
+
+def _fields(torch, shapes=SHAPES, seed0=0):
+    return [fields.bubble_field(s, seed=seed0 + k, nbubbles=5, dtype=torch.float32, device="cuda")
+            for k, s in enumerate(shapes)]
+
+
+def _check_against_per_field(torch, bt, fs, eps, wtype, shuffle, zerobits=0):
+    torch.cuda.synchronize()
+    for k, f in enumerate(fs):
+        rec, off = czw.compress_field(f, eps[k], wtype, zerobits, shuffle)
+        n = int(off[-1])
+        brec, boff = bt.field_records(k)
+        assert torch.equal(boff, off), k
+        assert torch.equal(brec, rec[:n]), k
+        dec = czw.decompress_field(rec, off, f.shape, torch.float32, wtype, shuffle)
+        assert torch.equal(bt.outputs[k].view(torch.int32), dec.view(torch.int32)), k
+
+
+@pytest.mark.parametrize("wtype,shuffle,zerobits", [(3, 0, 0), (1, 1, 0), (2, 0, 4)])
+@pytest.mark.parametrize("shape", [(0, 0), (0, 1), (3, 1), (5, 0)])
+def test_batch_equals_per_field(torch_gpu, wtype, shuffle, zerobits, shape):
+    torch = torch_gpu
+    groups, queue = shape
+    print(f"[case] batch wtype={wtype} shuffle={shuffle} zb={zerobits} groups={groups} queue={queue}", flush=True)
+    fs = _fields(torch)
+    bt = B.Batch(fs, EPS, wtype=wtype, zerobits=zerobits, shuffle=shuffle)
+    bt.set_launch(groups, queue)
+    bt.compress()
+    bt.decompress()
+    _check_against_per_field(torch, bt, fs, EPS, wtype, shuffle, zerobits)
+
+
+def test_batch_field_against_oracle(torch_gpu, oracle):
+    torch = torch_gpu
+    fs = _fields(torch, SHAPES[:3], seed0=11)
+    bt = B.Batch(fs, EPS[:3], wtype=3)
+    bt.set_launch(0, 1)
+    bt.compress()
+    bt.decompress()
+    torch.cuda.synchronize()
+    k = 2
+    fh = fs[k].cpu().numpy()
+    slots, nb = oracle.compress_field(fh, EPS[k], 3)
+    rec, off = (t.cpu().numpy() for t in bt.field_records(k))
+    for b in range(len(nb)):
+        o = int(off[b])
+        assert int(rec[o:o + 4].view(np.int32)[0]) == int(nb[b])
+        assert np.array_equal(rec[o + 4:o + 4 + int(nb[b])], slots[b, :nb[b]])
+    ref = oracle.decompress_field(slots, nb, fh.shape, 3, "f32")
+    assert np.array_equal(bt.outputs[k].cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+def test_batch_graph_replays(torch_gpu):
+    torch = torch_gpu
+    fs = _fields(torch, seed0=30)
+    bt = B.Batch(fs, EPS, wtype=1)
+    bt.set_launch(0, 1)
+    bt.compress()
+    bt.decompress()
+    torch.cuda.synchronize()
+    want_rec = bt.records[: int(bt.offsets[-1])].clone()
+    want_out = [o.clone() for o in bt.outputs]
+    g = bt.graph(("compress", "decompress"))
+    try:
+        for rep in range(3):
+            bt.records.zero_()
+            for o in bt.outputs:
+                o.fill_(float("nan"))
+            torch.cuda.synchronize()
+            g.launch()
+            torch.cuda.synchronize()
+            assert int(bt.err.item()) == 0
+            assert torch.equal(bt.records[: want_rec.numel()], want_rec), rep
+            for k in range(len(fs)):
+                assert torch.equal(bt.outputs[k].view(torch.int32), want_out[k].view(torch.int32)), (rep, k)
+            # eager work between replays
+            _ = sum(float(f.sum()) for f in fs)
+    finally:
+        g.close()
+
+
+def test_batch_autotune_keeps_results(torch_gpu):
+    torch = torch_gpu
+    fs = _fields(torch, seed0=50)
+    bt = B.Batch(fs, EPS, wtype=3)
+    ms_c = bt.autotune("compress")
+    ms_d = bt.autotune("decompress")
+    assert ms_c > 0 and ms_d > 0
+    assert bt.launch_shape["groups"] > 0
+    bt.compress()
+    bt.decompress()
+    _check_against_per_field(torch, bt, fs, EPS, 3, 0)
+
+
+def test_batch_corrupt_record_sets_error(torch_gpu):
+    torch = torch_gpu
+    fs = _fields(torch, SHAPES[:3], seed0=70)
+    bt = B.Batch(fs, EPS[:3], wtype=3)
+    bt.compress()
+    torch.cuda.synchronize()
+    b0 = int(bt.table[1]["block0"])
+    o = int(bt.offsets[b0])
+    bt.records[o:o + 4] = torch.tensor([255, 255, 255, 127], dtype=torch.uint8, device="cuda")
+    with pytest.raises(czw.CZWError):
+        bt.decompress()
