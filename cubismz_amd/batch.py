@@ -89,7 +89,7 @@ class Graph:
 
 
 class Batch:
-    """A set of float32 device fields (HDF5 order [nx][ny][nz]) with one eps each.
+    """A set of float32 (or float64) device fields (HDF5 order [nx][ny][nz]) with one eps each.
 
     ``compress()`` fills ``records``/``offsets`` for every field;
     ``decompress()`` writes every field's reconstruction into ``outputs``.
@@ -97,8 +97,13 @@ class Batch:
 
     def __init__(self, fields, eps, wtype=3, zerobits=0, shuffle=False, outputs=None):
         torch = czw._torch()
-        if any(f.dtype != torch.float32 or not f.is_cuda or not f.is_contiguous() or f.dim() != 3 for f in fields):
-            raise czw.CZWError("batch fields must be contiguous float32 device tensors [nx, ny, nz]")
+        if not fields:
+            raise czw.CZWError("empty batch")
+        dtype = fields[0].dtype
+        if dtype not in (torch.float32, torch.float64) or any(
+                f.dtype != dtype or not f.is_cuda or not f.is_contiguous() or f.dim() != 3 for f in fields):
+            raise czw.CZWError("batch fields must be contiguous device tensors [nx, ny, nz] of one dtype, "
+                               "float32 or float64")
         self.fields = list(fields)
         self.table, self.nblocks = prepare_table([tuple(f.shape) for f in fields], eps)
         dev = self.fields[0].device
@@ -107,7 +112,7 @@ class Batch:
             self.table[k]["src"] = f.data_ptr()
             self.table[k]["dst"] = self.outputs[k].data_ptr()
         self.table_dev = torch.from_numpy(self.table.view(np.uint8).copy()).to(dev)
-        self.prec = 4
+        self.prec = czw._prec(dtype, torch)
         self.workspace = torch.empty(_lib().czw_batch_workspace_bytes(self.nblocks, self.prec), dtype=torch.uint8,
                                      device=dev)
         self.records = torch.empty(czw.records_bound(self.nblocks, self.prec), dtype=torch.uint8, device=dev)

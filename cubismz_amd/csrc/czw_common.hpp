@@ -1,0 +1,297 @@
+// czw_common.hpp -- pieces shared by the binary32 and binary64 block
+// kernels: sizes, the batch field table, field geometry, LDS row helpers,
+// block-wide scans, the [0,16)^3 corner passes and stream pieces, and host
+// launch helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "../../include/czw.h"
+#include "czw_wavelet.hpp"
+
+namespace czw {
+
+constexpr int NB3 = 32768;
+constexpr int MASK_BYTES = 4096;
+constexpr int NT = 512;            // threads per block workgroup (8 wave64)
+constexpr int NROW = 1024 / NT;    // level-32 rows (and stream segments) per thread
+
+enum { LAYOUT_BLOCKS = 0, LAYOUT_FIELD = 1, LAYOUT_BATCH = 2 };
+constexpr int BATCH_MAX_FIELDS = 128;  // field table entries one launch keeps in LDS
+
+template <int N, typename R>
+CZW_DI void lds_ld(const R *A, int base, int stride, R *r)
+{
+#pragma unroll
+    for (int k = 0; k < N; ++k) r[k] = A[base + k * stride];
+}
+
+template <int N, typename R>
+CZW_DI void lds_st(R *A, int base, int stride, const R *r)
+{
+#pragma unroll
+    for (int k = 0; k < N; ++k) A[base + k * stride] = r[k];
+}
+
+// Wave64 inclusive prefix sum.
+CZW_DI uint32_t wave_incl_scan(uint32_t x, int lane)
+{
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+// Block-wide exclusive scan of one value per thread.  Returns the exclusive
+// prefix; *total receives the sum.  Ends with the wave totals read, so the
+// caller must barrier before wsum is written again.
+CZW_DI uint32_t block_excl_scan(uint32_t v, uint32_t *wsum, uint32_t *total)
+{
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t inc = wave_incl_scan(v, lane);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) {
+        const uint32_t s = wsum[i];
+        base += (i < w) ? s : 0u;
+        tot += s;
+    }
+    *total = tot;
+    return base + inc - v;
+}
+
+struct FieldGeom {
+    int nx, ny, nz;
+};
+
+// Global element offset of block b's row start (x = 0) for the x-row (y, z)
+// of an HDF5-ordered field, and the stride along x.
+CZW_DI size_t field_row(const FieldGeom &g, int b, int y, int z, size_t *xstride)
+{
+    const int bx = g.nx >> 5, by = g.ny >> 5;
+    const int ix = b % bx, iy = (b / bx) % by, iz = b / (bx * by);
+    *xstride = (size_t)g.ny * (size_t)g.nz;
+    return ((size_t)(32 * ix) * g.ny + (size_t)(32 * iy + y)) * g.nz + (size_t)(32 * iz + z);
+}
+
+// The block a workgroup works on, resolved from the batch-wide index b: the
+// field it belongs to (source or destination, geometry, threshold) and its
+// index lb inside that field.  Outside LAYOUT_BATCH this is the launch's one
+// field and lb = b.
+template <typename R>
+struct BlockRef {
+    const R *src;
+    R *dst;
+    FieldGeom g;
+    int lb;
+    float eps;
+};
+
+// Field table of a batch launch, copied into LDS once per workgroup.
+template <int LAYOUT>
+CZW_DI void load_table(czw_field_t *T, const czw_field_t *__restrict__ fields, int nf)
+{
+    if constexpr (LAYOUT == LAYOUT_BATCH) {
+        constexpr int W = sizeof(czw_field_t) / 4;
+        const uint32_t *s = reinterpret_cast<const uint32_t *>(fields);
+        uint32_t *d = reinterpret_cast<uint32_t *>(T);
+        for (int i = threadIdx.x; i < nf * W; i += blockDim.x) d[i] = s[i];
+        __syncthreads();
+    }
+}
+
+template <int LAYOUT, typename R>
+CZW_DI BlockRef<R> locate(const czw_field_t *T, int nf, int b, const R *src, R *dst, FieldGeom g, float eps)
+{
+    if constexpr (LAYOUT != LAYOUT_BATCH) {
+        return BlockRef<R>{src, dst, g, b, eps};
+    } else {
+        int lo = 0, hi = nf - 1;  // the last field whose first block is <= b
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (T[mid].block0 <= b) lo = mid;
+            else hi = mid - 1;
+        }
+        const czw_field_t &f = T[lo];
+        return BlockRef<R>{static_cast<const R *>(f.src), static_cast<R *>(f.dst), FieldGeom{f.nx, f.ny, f.nz},
+                           b - f.block0, f.eps};
+    }
+}
+
+// Load/store layout of a block's rows: batch fields are HDF5-ordered fields.
+template <int LAYOUT>
+constexpr int row_layout()
+{
+    return LAYOUT == LAYOUT_BLOCKS ? LAYOUT_BLOCKS : LAYOUT_FIELD;
+}
+
+// Remaining forward passes on the [0,16)^3 corner of a block held in LDS at
+// z*SP_ + y*RP_ + x: level 16 (x, y) and level 8 (y fused, z, x).
+template <int WT, int RP_, int SP_, typename R>
+CZW_DI void fwt16_8(R *A, int t)
+{
+    R r[16];
+    if (t < 256) {  // level 16 x-pass: row (z, y) in [0,16)^2
+        const int z = t >> 4, y = t & 15;
+        lds_ld<16>(A, z * SP_ + y * RP_, 1, r);
+        row<16, true, WT, R>(r);
+        lds_st<16>(A, z * SP_ + y * RP_, 1, r);
+    }
+    __syncthreads();
+    if (t < 256) {  // level 16 y-pass, then level 8 y-pass: row (z, x)
+        const int z = t >> 4, x = t & 15;
+        lds_ld<16>(A, z * SP_ + x, RP_, r);
+        row<16, true, WT, R>(r);
+        if (z < 8 && x < 8) row<8, true, WT, R>(r);
+        lds_st<16>(A, z * SP_ + x, RP_, r);
+    }
+    __syncthreads();
+    if (t < 64) {  // level 8 z-pass: row (y, x) in [0,8)^2
+        const int y = t >> 3, x = t & 7;
+        lds_ld<8>(A, y * RP_ + x, SP_, r);
+        row<8, true, WT, R>(r);
+        lds_st<8>(A, y * RP_ + x, SP_, r);
+    }
+    __syncthreads();
+    if (t < 64) {  // level 8 x-pass: row (z, y)
+        const int z = t >> 3, y = t & 7;
+        lds_ld<8>(A, z * SP_ + y * RP_, 1, r);
+        row<8, true, WT, R>(r);
+        lds_st<8>(A, z * SP_ + y * RP_, 1, r);
+    }
+    __syncthreads();
+}
+
+// Inverse passes of all levels except the final level-32 x-pass, which the
+// caller fuses with the store: level 8 (x, z, y), level 16 (y fused, x, z),
+// level 32 (z fused, y).
+template <int WT, int RP_, int SP_, typename R>
+CZW_DI void iwt16_8(R *A, int t)
+{
+    {
+        R r[16];
+        if (t < 64) {  // level 8 x-pass
+            const int z = t >> 3, y = t & 7;
+            lds_ld<8>(A, z * SP_ + y * RP_, 1, r);
+            row<8, false, WT, R>(r);
+            lds_st<8>(A, z * SP_ + y * RP_, 1, r);
+        }
+        __syncthreads();
+        if (t < 64) {  // level 8 z-pass
+            const int y = t >> 3, x = t & 7;
+            lds_ld<8>(A, y * RP_ + x, SP_, r);
+            row<8, false, WT, R>(r);
+            lds_st<8>(A, y * RP_ + x, SP_, r);
+        }
+        __syncthreads();
+        if (t < 256) {  // level 8 y-pass, then level 16 y-pass
+            const int z = t >> 4, x = t & 15;
+            lds_ld<16>(A, z * SP_ + x, RP_, r);
+            if (z < 8 && x < 8) row<8, false, WT, R>(r);
+            row<16, false, WT, R>(r);
+            lds_st<16>(A, z * SP_ + x, RP_, r);
+        }
+        __syncthreads();
+        if (t < 256) {  // level 16 x-pass
+            const int z = t >> 4, y = t & 15;
+            lds_ld<16>(A, z * SP_ + y * RP_, 1, r);
+            row<16, false, WT, R>(r);
+            lds_st<16>(A, z * SP_ + y * RP_, 1, r);
+        }
+        __syncthreads();
+    }
+}
+
+// Stream positions [8p, 8p+8) of the [0,16)^3 corner, p < 512 (see
+// for_each_run): two runs of 4 for p < 64 (base 4^3 and level-8 details),
+// one run of 8 otherwise (level-16 details).
+template <int RP_, int SP_, typename F>
+CZW_DI void corner_piece(int p, F &&f)
+{
+    if (p < 64) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int s = 8 * p + 4 * q;
+            const int code = s >> 6, k = s & 63;
+            const int a0 = 4 * ((code >> 2) & 1) + (k >> 4);
+            const int a1 = 4 * ((code >> 1) & 1) + ((k >> 2) & 3);
+            const int o2 = 4 * (code & 1);
+            f(q, 4, a0 * SP_ + a1 * RP_ + o2, 1, a1 + 32 * a0, o2);
+        }
+    } else {
+        const int s = 8 * p;
+        const int code = s >> 9, k = s & 511;
+        const int a0 = 8 * ((code >> 2) & 1) + (k >> 6);
+        const int a1 = 8 * ((code >> 1) & 1) + ((k >> 3) & 7);
+        const int o2 = 8 * (code & 1);
+        f(0, 8, a1 * SP_ + o2 * RP_ + a0, RP_, a1 + 32 * a0, o2);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+
+static int wt_index(int wtype)
+{
+    switch (wtype) {
+    case 1: return 1;
+    case 2: return 2;
+    case 0:
+    case 3: return 3;
+    default: return -1;
+    }
+}
+
+// Workgroups that fit on the current device at once for a kernel with the
+// given block size (the persistent kernels use one ~140 KB LDS block per CU).
+static int resident_groups(const void *kernel, int threads)
+{
+    static int cus[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev] = n;
+    }
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per <= 0) per = 1;
+    return cus[dev] * per;
+}
+
+static bool zb_ok(int zb) { return zb == 0 || zb == 4 || zb == 8 || zb == 12 || zb == 16; }
+
+static bool field_ok(int nx, int ny, int nz)
+{
+    return nx > 0 && ny > 0 && nz > 0 && !(nx & 31) && !(ny & 31) && !(nz & 31) &&
+           (size_t)(nx / 32) * (ny / 32) * (nz / 32) < (size_t)INT32_MAX;
+}
+
+// Exclusive scan of record sizes 4 + nbytes[b] into offsets[0..n] (one
+// workgroup); defined with the binary32 kernels.
+int scan_records(const uint32_t *nbytes, int n, uint64_t *offsets, hipStream_t s);
+
+// Per-block size words after the slots of a compressor workspace: the payload
+// size, then the region counts the packer needs (binary32: the corner count;
+// binary64: the corner and level-32 code counts).
+inline size_t size_words(int prec) { return prec == 8 ? 9 : 2; }
+
+// binary64 kernels (czw_f64.hip); layout is LAYOUT_BLOCKS, LAYOUT_FIELD or
+// LAYOUT_BATCH (then src/dst come from the field table); groups = 0 launches
+// every resident workgroup; queue = nullptr keeps the static block order.
+int compress_slots_f64(int layout, const void *src, int nblocks, FieldGeom g, float eps, int wtype, int zerobits,
+                       uint8_t *slots, uint32_t *nbytes, const czw_field_t *fields, int nf, int groups,
+                       uint32_t *queue, hipStream_t s);
+int pack_records_f64(const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle, uint8_t *records,
+                     uint64_t *offsets, hipStream_t s);
+int decompress_f64(int layout, const void *records, size_t rec_bytes, const uint64_t *offsets, int nblocks,
+                   FieldGeom g, int wtype, int shuffle, void *dst, uint32_t *err, const czw_field_t *fields, int nf,
+                   int groups, uint32_t *queue, hipStream_t s);
+
+}  // namespace czw

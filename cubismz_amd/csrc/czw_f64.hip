@@ -1,0 +1,627 @@
+// czw_f64.hip -- binary64 block compressor and decompressor: the reference's
+// double build (Real = double; WaveletCompressor.cpp:153-259 with
+// sizeof(DataType) = 8).
+//
+// A 32^3 block of doubles is 256 KB and does not fit in LDS (160 KB), so a
+// workgroup keeps only one x-half of the block there at a time:
+//
+//   compress:   level-32 x-pass on the thread's two x-rows in registers;
+//               x >= 16 half -> LDS: y-pass, z-pass, threshold, survivors
+//               (all final: level-32 codes 2, 3, 6, 7);
+//               x <  16 half -> LDS: y-pass, z-pass (+ level-16 z-pass on the
+//               [0,16)^2 columns), survivors of codes 1, 4, 5;
+//               the [0,16)^3 corner stays in LDS for levels 16 and 8.
+//   decompress: the mirror image, x < 16 half first (it holds the corner),
+//               then the x >= 16 half, then the level-32 x-pass in registers.
+//
+// Because the two halves are handled one after the other, the survivors of
+// each half go to their own region of the block's slot with per-code counts
+// (see SLOT64 below); the packing kernel concatenates the regions in stream
+// order (codes 0..7, FullWaveletTransform.h:59-96).  The arithmetic is plain
+// binary64 without contraction (-ffp-contract=off), as in the reference.
+//
+// LDS layout of a half: H[z][y][x & 15] at z*SPD + y*RPD + (x & 15) with
+// RPD = 17, SPD = 545 (doubles): x-rows for either thread order, y- and
+// z-rows and the corner passes are at most 2-way bank conflicted.
+#include "czw_common.hpp"
+
+namespace czw {
+
+constexpr int RPD = 17;
+constexpr int SPD = 32 * RPD + 1;
+constexpr int HALF_WORDS = 32 * SPD;  // doubles
+constexpr size_t SLOT64 = MASK_BYTES + 8 * (size_t)NB3;
+// Survivor regions of a slot, in doubles after the 4096-byte mask: the
+// [0,16)^3 corner (stream positions [0,4096), <= 4096 values), then the
+// x < 16 half's level-32 codes 1, 4, 5 (<= 3 x 4096), then the x >= 16
+// half's codes 2, 3, 6, 7 (<= 4 x 4096): 8 x 4096 doubles in all.
+constexpr int REG_CORNER = 0;
+constexpr int REG_H0 = 4096;
+constexpr int REG_H1 = 4 * 4096;
+constexpr int NCNT = 8;  // per-block counts: corner, then level-32 codes 1..7
+
+// x-row q (0..1023) of a block: (z, y) = (q >> 5, q & 31) in the block
+// layout, (y, z) = (q >> 5, q & 31) in the HDF5 field layout (as for
+// binary32); its x-half in H starts here.
+template <int RL>
+CZW_DI int xrow_half_base(int q)
+{
+    if constexpr (RL == LAYOUT_BLOCKS) return (q >> 5) * SPD + (q & 31) * RPD;
+    else return (q & 31) * SPD + (q >> 5) * RPD;
+}
+
+template <int RL>
+CZW_DI void load_xrow_f64(const double *__restrict__ src, const FieldGeom &g, int b, int q, double *r)
+{
+    if constexpr (RL == LAYOUT_BLOCKS) {
+        const double2 *p = reinterpret_cast<const double2 *>(src + (size_t)b * NB3 + (size_t)q * 32);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const double2 v = p[i];
+            r[2 * i] = v.x;
+            r[2 * i + 1] = v.y;
+        }
+    } else {
+        size_t xs;
+        const double *p = src + field_row(g, b, q >> 5, q & 31, &xs);
+#pragma unroll
+        for (int x = 0; x < 32; ++x) r[x] = p[x * xs];
+    }
+}
+
+template <int RL>
+CZW_DI void store_xrow_f64(double *__restrict__ dst, const FieldGeom &g, int b, int q, const double *r)
+{
+    if constexpr (RL == LAYOUT_BLOCKS) {
+        double2 *p = reinterpret_cast<double2 *>(dst + (size_t)b * NB3 + (size_t)q * 32);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) p[i] = make_double2(r[2 * i], r[2 * i + 1]);
+    } else {
+        size_t xs;
+        double *p = dst + field_row(g, b, q >> 5, q & 31, &xs);
+#pragma unroll
+        for (int x = 0; x < 32; ++x) p[x * xs] = r[x];
+    }
+}
+
+// float_zero_bits3 (WaveletCompressor.cpp:34-43) on the first 32-bit word of
+// a little-endian double: the low mantissa word.
+CZW_DI double zero_low_bits(double v, uint32_t zkeep)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return __longlong_as_double((long long)(u & (0xffffffff00000000ull | (unsigned long long)zkeep)));
+}
+
+// Exclusive scan of cnt[0..1024) in place (two entries per thread); returns
+// the total.  Barriers before and after are the caller's.
+CZW_DI uint32_t scan1024(uint32_t *cnt, uint32_t *wsum, int t)
+{
+    const uint32_t c0 = cnt[2 * t], c1 = cnt[2 * t + 1];
+    uint32_t total;
+    const uint32_t run = block_excl_scan(c0 + c1, wsum, &total);
+    cnt[2 * t] = run;
+    cnt[2 * t + 1] = run + c0;
+    return total;
+}
+
+// ---------------------------------------------------------------------------
+// Compressor.  Persistent like the binary32 one (static stride or work queue,
+// LAYOUT_BATCH field tables).  Block b's survivors go to the regions of slot
+// b, its payload size to nbytes[b] and its 8 region counts to
+// counts[8b .. 8b+8).
+template <int WT, int LAYOUT>
+__global__ __launch_bounds__(NT) void k_compress_f64(const double *__restrict__ src, FieldGeom g, int nblocks,
+                                                     float eps, uint32_t zkeep, uint8_t *__restrict__ slots,
+                                                     uint32_t *__restrict__ nbytes, uint32_t *__restrict__ counts,
+                                                     const czw_field_t *__restrict__ fields, int nf,
+                                                     uint32_t *__restrict__ queue)
+{
+    constexpr int RL = row_layout<LAYOUT>();
+    __shared__ double H[HALF_WORDS];
+    __shared__ uint32_t M[1024];
+    __shared__ uint32_t cnt[1024];
+    __shared__ uint32_t wsum[NT / 64];
+    __shared__ int ticket[2];
+    __shared__ czw_field_t T[LAYOUT == LAYOUT_BATCH ? BATCH_MAX_FIELDS : 1];
+    const int t = threadIdx.x;
+    const int G = gridDim.x;
+    load_table<LAYOUT>(T, fields, nf);
+    // pass-row owner: (slow, x & 15) = (t >> 4, t & 15), slow = z (y-pass) or y (z-pass)
+    const int xh = t & 15, sl = t >> 4;
+    for (int b = blockIdx.x, it = 0; b < nblocks; ++it) {
+        if (queue && t == 0) ticket[it & 1] = G + (int)atomicAdd(queue, 1u);
+        const BlockRef<double> br = locate<LAYOUT, double>(T, nf, b, src, nullptr, g, eps);
+        const double e = (double)br.eps;  // the double build compares against (double)eps
+        uint8_t *slot = slots + (size_t)b * SLOT64;
+        double *reg = reinterpret_cast<double *>(slot + MASK_BYTES);
+        double xr[NROW][32];
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) {  // level 32, x-pass, in registers
+            load_xrow_f64<RL>(br.src, br.g, br.lb, t + NT * k, xr[k]);
+            row<32, true, WT, double>(xr[k]);
+        }
+
+        // ---- x >= 16: level-32 y- and z-passes; every coefficient is final
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) lds_st<16>(H, xrow_half_base<RL>(t + NT * k), 1, &xr[k][16]);
+        __syncthreads();
+        {
+            double r[32];
+            lds_ld<32>(H, sl * SPD + xh, RPD, r);  // y-pass: row (z, x)
+            row<32, true, WT, double>(r);
+            lds_st<32>(H, sl * SPD + xh, RPD, r);
+        }
+        __syncthreads();
+        uint32_t S1;
+        {
+            const int y = sl, x = 16 + xh;
+            double r[32];
+            lds_ld<32>(H, y * RPD + xh, SPD, r);  // z-pass: row (y, x)
+            row<32, true, WT, double>(r);
+            uint32_t bits = 0;
+#pragma unroll
+            for (int z = 0; z < 32; ++z) bits |= (uint32_t)(fabs(r[z]) > e) << z;
+            M[x + 32 * y] = bits;
+            // region order: codes 2, 3 (y < 16), 6, 7 (y >= 16); code bit 0 = z >= 16
+            const int ri = 2 * (y >> 4), base = (y & 15) * 16 + xh;
+            cnt[ri * 256 + base] = __popc(bits & 0xffffu);
+            cnt[(ri + 1) * 256 + base] = __popc(bits >> 16);
+            __syncthreads();
+            S1 = scan1024(cnt, wsum, t);
+            __syncthreads();
+            if (t == 0) {
+                uint32_t *cb = counts + (size_t)NCNT * b;
+                cb[2] = cnt[256] - cnt[0];
+                cb[3] = cnt[512] - cnt[256];
+                cb[6] = cnt[768] - cnt[512];
+                cb[7] = S1 - cnt[768];
+            }
+            uint32_t i0 = cnt[ri * 256 + base], i1 = cnt[(ri + 1) * 256 + base];
+#pragma unroll
+            for (int z = 0; z < 32; ++z) {
+                uint32_t &i = z < 16 ? i0 : i1;
+                if ((bits >> z) & 1u) reg[REG_H1 + i++] = zero_low_bits(r[z], zkeep);
+            }
+        }
+        __syncthreads();  // H and cnt are free again
+
+        // ---- x < 16: level-32 y- and z-passes, level-16 z-pass on the corner columns
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) lds_st<16>(H, xrow_half_base<RL>(t + NT * k), 1, &xr[k][0]);
+        __syncthreads();
+        {
+            double r[32];
+            lds_ld<32>(H, sl * SPD + xh, RPD, r);
+            row<32, true, WT, double>(r);
+            lds_st<32>(H, sl * SPD + xh, RPD, r);
+        }
+        __syncthreads();
+        uint32_t S0;
+        {
+            const int y = sl, x = xh;
+            const bool corner = y < 16;
+            double r[32];
+            lds_ld<32>(H, y * RPD + xh, SPD, r);
+            row<32, true, WT, double>(r);
+            uint32_t bits = 0;
+#pragma unroll
+            for (int z = 0; z < 32; ++z) bits |= (uint32_t)(fabs(r[z]) > e) << z;
+            if (corner) {
+                double c[16];
+#pragma unroll
+                for (int z = 0; z < 16; ++z) c[z] = r[z];
+                row<16, true, WT, double>(c);
+                lds_st<16>(H, y * RPD + xh, SPD, c);
+                bits &= 0xffff0000u;  // z < 16 is decided with the corner below
+            }
+            M[x + 32 * y] = bits;
+            // region order: code 1 (y < 16, z >= 16), code 4 (y >= 16, z < 16), code 5
+            const int base = (y & 15) * 16 + xh;
+            if (corner) {
+                cnt[base] = __popc(bits >> 16);
+            } else {
+                cnt[256 + base] = __popc(bits & 0xffffu);
+                cnt[512 + base] = __popc(bits >> 16);
+            }
+            if (t < 256) cnt[768 + t] = 0u;
+            __syncthreads();
+            S0 = scan1024(cnt, wsum, t);
+            __syncthreads();
+            if (t == 0) {
+                uint32_t *cb = counts + (size_t)NCNT * b;
+                cb[1] = cnt[256] - cnt[0];
+                cb[4] = cnt[512] - cnt[256];
+                cb[5] = S0 - cnt[512];
+            }
+            uint32_t i0 = corner ? 0u : cnt[256 + base], i1 = corner ? cnt[base] : cnt[512 + base];
+#pragma unroll
+            for (int z = 0; z < 32; ++z) {
+                uint32_t &i = z < 16 ? i0 : i1;
+                if ((bits >> z) & 1u) reg[REG_H0 + i++] = zero_low_bits(r[z], zkeep);
+            }
+        }
+        // levels 16 and 8 on the corner (the barrier inside orders the column stores above)
+        __syncthreads();
+        fwt16_8<WT, RPD, SPD>(H, t);
+
+        // the [0,16)^3 corner: stream positions [0,4096) in pieces of 8 (base 4^3 always kept)
+        uint32_t keep = 0;
+        double v[8];
+        corner_piece<RPD, SPD>(t, [&](int q, int h, int addr, int stride, int word, int shift) {
+            uint32_t bits = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (i < h) {
+                    const double c = H[addr + i * stride];
+                    v[q * h + i] = c;
+                    bits |= (uint32_t)((t < 8) || (fabs(c) > e)) << i;
+                }
+            }
+            keep |= bits << (q * h);
+            atomicOr(&M[word], bits << shift);
+        });
+        uint32_t Sc;
+        uint32_t idx = block_excl_scan(__popc(keep), wsum, &Sc);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if ((keep >> j) & 1u) reg[REG_CORNER + idx++] = zero_low_bits(v[j], zkeep);
+        __syncthreads();  // mask complete
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) reinterpret_cast<uint32_t *>(slot)[t + NT * k] = M[t + NT * k];
+        if (t == 0) {
+            counts[(size_t)NCNT * b] = Sc;
+            nbytes[b] = MASK_BYTES + 8u * (Sc + S0 + S1);
+        }
+        const int bnext = queue ? ticket[it & 1] : b + G;
+        __syncthreads();  // M, cnt, wsum and ticket are read before the next block reuses them
+        b = bnext;
+    }
+}
+
+// Frame each block's payload into the record stream: size word, mask, then
+// the survivors in stream order (corner, codes 1..7 from their regions), with
+// the byte-plane shuffle for sizeof(Real) = 8 if asked (WaveletCompressor.cpp:56-73).
+// Records start at 4-byte offsets, so doubles are written as two words.
+__global__ __launch_bounds__(256) void k_pack_records_f64(const uint8_t *__restrict__ slots,
+                                                          const uint32_t *__restrict__ nbytes,
+                                                          const uint32_t *__restrict__ counts, int shuffle,
+                                                          const uint64_t *__restrict__ offsets,
+                                                          uint8_t *__restrict__ records)
+{
+    const int b = blockIdx.x, t = threadIdx.x;
+    const uint32_t n = nbytes[b];
+    const uint32_t S = (n - MASK_BYTES) / 8;
+    const uint32_t *cb = counts + (size_t)NCNT * b;
+    uint32_t c[NCNT];
+#pragma unroll
+    for (int i = 0; i < NCNT; ++i) c[i] = cb[i];
+    // region start of each stream segment (corner, codes 1..7)
+    const uint32_t from[NCNT] = {(uint32_t)REG_CORNER, (uint32_t)REG_H0, (uint32_t)REG_H1, REG_H1 + c[2],
+                                 REG_H0 + c[1], REG_H0 + c[1] + c[4], REG_H1 + c[2] + c[3],
+                                 REG_H1 + c[2] + c[3] + c[6]};
+    uint32_t st[NCNT + 1];
+    st[0] = 0;
+#pragma unroll
+    for (int i = 0; i < NCNT; ++i) st[i + 1] = st[i] + c[i];
+    const uint8_t *slot = slots + (size_t)b * SLOT64;
+    const unsigned long long *reg = reinterpret_cast<const unsigned long long *>(slot + MASK_BYTES);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(records + offsets[b]);
+    uint8_t *dst8 = records + offsets[b] + 4 + MASK_BYTES;
+    if (t == 0) dst[0] = n;
+    for (uint32_t i = t; i < MASK_BYTES / 4; i += 256) dst[1 + i] = reinterpret_cast<const uint32_t *>(slot)[i];
+    for (uint32_t i = t; i < S; i += 256) {
+        int seg = 0;
+#pragma unroll
+        for (int j = 1; j < NCNT; ++j) seg += (i >= st[j]) ? 1 : 0;
+        const unsigned long long v = reg[from[seg] + (i - st[seg])];
+        if (shuffle) {
+#pragma unroll
+            for (int p = 0; p < 8; ++p) dst8[(size_t)p * S + i] = (uint8_t)(v >> (8 * p));
+        } else {
+            dst[1 + MASK_BYTES / 4 + 2 * i] = (uint32_t)v;
+            dst[1 + MASK_BYTES / 4 + 2 * i + 1] = (uint32_t)(v >> 32);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Decompressor (WaveletCompressor.cpp:232-259 with FullWaveletTransform.h:
+// 100-150 and the inverse transform), one block per workgroup iteration.
+// Survivors are read straight from the record (they are at 4-byte offsets;
+// byte planes when shuffled).
+struct Payload64 {
+    const uint32_t *w;  // unshuffled: survivor i = words 2i, 2i+1
+    const uint8_t *b8;  // shuffled: byte p of survivor i at p*S + i
+    uint32_t S;
+    int shuffle;
+    CZW_DI double get(uint32_t i) const
+    {
+        unsigned long long u;
+        if (shuffle) {
+            u = 0;
+#pragma unroll
+            for (int p = 0; p < 8; ++p) u |= (unsigned long long)b8[(size_t)p * S + i] << (8 * p);
+        } else {
+            u = (unsigned long long)w[2 * i] | ((unsigned long long)w[2 * i + 1] << 32);
+        }
+        return __longlong_as_double((long long)u);
+    }
+};
+
+template <int WT, int LAYOUT>
+__global__ __launch_bounds__(NT) void k_decompress_f64(const uint8_t *__restrict__ records, uint64_t rec_bytes,
+                                                       const uint64_t *__restrict__ offsets, int nblocks,
+                                                       int shuffle, double *__restrict__ dst, FieldGeom g,
+                                                       uint32_t *__restrict__ err,
+                                                       const czw_field_t *__restrict__ fields, int nf,
+                                                       uint32_t *__restrict__ queue)
+{
+    constexpr int RL = row_layout<LAYOUT>();
+    __shared__ double H[HALF_WORDS];
+    __shared__ uint32_t M[1024];
+    __shared__ uint32_t cnt[2560];  // corner pieces [0,512), level-32 runs 512 + (code-1)*256 + run
+    __shared__ uint32_t wsum[NT / 64];
+    __shared__ uint32_t flag;
+    __shared__ int ticket[2];
+    __shared__ czw_field_t T[LAYOUT == LAYOUT_BATCH ? BATCH_MAX_FIELDS : 1];
+    const int t = threadIdx.x;
+    const int G = gridDim.x;
+    load_table<LAYOUT>(T, fields, nf);
+    const int xh = t & 15, sl = t >> 4;
+    int bn = 0;
+    for (int b = blockIdx.x, it = 0; b < nblocks; ++it, b = bn) {
+        if (queue) {
+            if (t == 0) ticket[it & 1] = G + (int)atomicAdd(queue, 1u);
+            __syncthreads();
+            bn = ticket[it & 1];
+        } else {
+            bn = b + G;
+        }
+        const uint64_t ob = offsets[b];
+        if ((ob & 3u) || ob + 4 + MASK_BYTES > rec_bytes) {
+            if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_NBYTES);
+            continue;
+        }
+        const uint32_t *p = reinterpret_cast<const uint32_t *>(records + ob);
+        const uint32_t nb = p[0];
+        if (nb < MASK_BYTES || ((nb - MASK_BYTES) & 7u) || nb > SLOT64 || ob + 4 + nb > rec_bytes) {
+            if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_NBYTES);
+            continue;
+        }
+        const uint32_t S = (nb - MASK_BYTES) >> 3;
+        const Payload64 pay{p + 1 + MASK_BYTES / 4, records + ob + 4 + MASK_BYTES, S, shuffle};
+        __syncthreads();  // the previous block's last reads of H, M, cnt and flag are done
+        if (t == 0) flag = 1u;
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) M[t + NT * k] = p[1 + t + NT * k];
+        __syncthreads();
+        // the 4^3 base is always stored (FullWaveletTransform.h:163-176)
+        if (t < 16 && (M[(t & 3) + 32 * (t >> 2)] & 0xfu) != 0xfu) flag = 0u;
+        uint32_t keep = 0;
+        corner_piece<RPD, SPD>(t, [&](int q, int h, int, int, int word, int shift) {
+            const uint32_t hm = (1u << h) - 1u;
+            keep |= ((t < 8) ? hm : ((M[word] >> shift) & hm)) << (q * h);
+        });
+        cnt[t] = __popc(keep);
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) {
+            const int q = t + NT * k, y = q >> 5, x = q & 31;
+            const bool corner = (y < 16) && (x < 16);
+            const uint32_t mw = M[x + 32 * y] & (corner ? 0xffff0000u : 0xffffffffu);
+            const int c0 = 2 * (x >> 4) + 4 * (y >> 4), base = (y & 15) * 16 + (x & 15);
+            if (!corner) cnt[512 + (c0 - 1) * 256 + base] = __popc(mw & 0xffffu);
+            cnt[512 + c0 * 256 + base] = __popc(mw >> 16);
+        }
+        if (t < 2560 - 512 - 1792) cnt[512 + 1792 + t] = 0u;
+        __syncthreads();
+        {  // exclusive scan of the 2560 counts in stream order, 5 per thread
+            uint32_t c5[5], sum = 0;
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                c5[i] = cnt[5 * t + i];
+                sum += c5[i];
+            }
+            uint32_t tot;
+            uint32_t run = block_excl_scan(sum, wsum, &tot);
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                cnt[5 * t + i] = run;
+                run += c5[i];
+            }
+            // with all base bits set, the kept count equals the mask popcount
+            // that WaveletCompressor.cpp:235-251 checks against the payload size
+            if (tot != S) flag = 0u;
+        }
+        __syncthreads();
+        if (!flag) {
+            if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_POPCOUNT);
+            continue;
+        }
+        {  // corner positions: survivor or +0
+            uint32_t idx = cnt[t];
+            corner_piece<RPD, SPD>(t, [&](int q, int h, int addr, int stride, int, int) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    if (i < h) {
+                        const bool bit = (keep >> (q * h + i)) & 1u;
+                        H[addr + i * stride] = bit ? pay.get(idx) : 0.0;
+                        idx += bit;
+                    }
+                }
+            });
+        }
+        __syncthreads();
+        iwt16_8<WT, RPD, SPD>(H, t);  // level 8 (x, z, y), level 16 (y, x)
+
+        // ---- x < 16: level-16 z-pass on the corner columns, level-32 z- and y-passes
+        {
+            const int y = sl, x = xh, base = (y & 15) * 16 + xh;
+            const uint32_t mw = M[x + 32 * y];
+            double r[32];
+            if (y < 16) {
+                double c[16];
+                lds_ld<16>(H, y * RPD + xh, SPD, c);
+                row<16, false, WT, double>(c);
+#pragma unroll
+                for (int z = 0; z < 16; ++z) r[z] = c[z];
+                uint32_t i1 = cnt[512 + base];  // code 1
+#pragma unroll
+                for (int z = 16; z < 32; ++z) {
+                    const bool bit = (mw >> z) & 1u;
+                    r[z] = bit ? pay.get(i1) : 0.0;
+                    i1 += bit;
+                }
+            } else {
+                uint32_t i0 = cnt[512 + 3 * 256 + base], i1 = cnt[512 + 4 * 256 + base];  // codes 4, 5
+#pragma unroll
+                for (int z = 0; z < 32; ++z) {
+                    uint32_t &i = z < 16 ? i0 : i1;
+                    const bool bit = (mw >> z) & 1u;
+                    r[z] = bit ? pay.get(i) : 0.0;
+                    i += bit;
+                }
+            }
+            row<32, false, WT, double>(r);
+            lds_st<32>(H, y * RPD + xh, SPD, r);
+        }
+        __syncthreads();
+        {
+            double r[32];
+            lds_ld<32>(H, sl * SPD + xh, RPD, r);  // y-pass: row (z, x)
+            row<32, false, WT, double>(r);
+            lds_st<32>(H, sl * SPD + xh, RPD, r);
+        }
+        __syncthreads();
+        double lo[NROW][16];
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) lds_ld<16>(H, xrow_half_base<RL>(t + NT * k), 1, lo[k]);
+        __syncthreads();
+
+        // ---- x >= 16: level-32 z- and y-passes (codes 2, 3 for y < 16; 6, 7 for y >= 16)
+        {
+            const int y = sl, x = 16 + xh, base = (y & 15) * 16 + xh;
+            const int c0 = 2 + 4 * (y >> 4);
+            const uint32_t mw = M[x + 32 * y];
+            uint32_t i0 = cnt[512 + (c0 - 1) * 256 + base], i1 = cnt[512 + c0 * 256 + base];
+            double r[32];
+#pragma unroll
+            for (int z = 0; z < 32; ++z) {
+                uint32_t &i = z < 16 ? i0 : i1;
+                const bool bit = (mw >> z) & 1u;
+                r[z] = bit ? pay.get(i) : 0.0;
+                i += bit;
+            }
+            row<32, false, WT, double>(r);
+            lds_st<32>(H, y * RPD + xh, SPD, r);
+        }
+        __syncthreads();
+        {
+            double r[32];
+            lds_ld<32>(H, sl * SPD + xh, RPD, r);
+            row<32, false, WT, double>(r);
+            lds_st<32>(H, sl * SPD + xh, RPD, r);
+        }
+        __syncthreads();
+        // level-32 x-pass on whole rows in registers, then the store
+        const BlockRef<double> br = locate<LAYOUT, double>(T, nf, b, nullptr, dst, g, 0.0f);
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) {
+            const int q = t + NT * k;
+            double r[32];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) r[i] = lo[k][i];
+            lds_ld<16>(H, xrow_half_base<RL>(q), 1, &r[16]);
+            row<32, false, WT, double>(r);
+            store_xrow_f64<RL>(br.dst, br.g, br.lb, q, r);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+template <int LAYOUT>
+static int launch_compress_f64(int wt, const double *src, FieldGeom g, int nblocks, float eps, uint32_t zkeep,
+                               uint8_t *slots, uint32_t *nb, uint32_t *cnt, const czw_field_t *fields, int nf,
+                               int groups, uint32_t *queue, hipStream_t s)
+{
+    const void *fn = wt == 1 ? (const void *)k_compress_f64<1, LAYOUT>
+                   : wt == 2 ? (const void *)k_compress_f64<2, LAYOUT> : (const void *)k_compress_f64<3, LAYOUT>;
+    const dim3 grid(std::min(nblocks, groups ? groups : resident_groups(fn, NT))), blk(NT);
+    if (wt == 1) hipLaunchKernelGGL((k_compress_f64<1, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, cnt, fields, nf, queue);
+    if (wt == 2) hipLaunchKernelGGL((k_compress_f64<2, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, cnt, fields, nf, queue);
+    if (wt == 3) hipLaunchKernelGGL((k_compress_f64<3, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, cnt, fields, nf, queue);
+    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
+}
+
+int compress_slots_f64(int layout, const void *src, int nblocks, FieldGeom g, float eps, int wtype, int zerobits,
+                       uint8_t *slots, uint32_t *nbytes, const czw_field_t *fields, int nf, int groups,
+                       uint32_t *queue, hipStream_t s)
+{
+    const int wt = wt_index(wtype);
+    if (wt < 0 || !zb_ok(zerobits) || nblocks <= 0 || !slots || !nbytes) return CZW_EINVAL;
+    if (layout != LAYOUT_BATCH && !src) return CZW_EINVAL;
+    const uint32_t zkeep = zerobits ? ~((1u << zerobits) - 1u) : 0xffffffffu;
+    uint32_t *cnt = nbytes + nblocks;
+    const double *d = static_cast<const double *>(src);
+    switch (layout) {
+    case LAYOUT_BLOCKS:
+        return launch_compress_f64<LAYOUT_BLOCKS>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, nullptr, 0,
+                                                  groups, queue, s);
+    case LAYOUT_FIELD:
+        return launch_compress_f64<LAYOUT_FIELD>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, nullptr, 0,
+                                                 groups, queue, s);
+    default:
+        return launch_compress_f64<LAYOUT_BATCH>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, fields, nf,
+                                                 groups, queue, s);
+    }
+}
+
+int pack_records_f64(const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle, uint8_t *records,
+                     uint64_t *offsets, hipStream_t s)
+{
+    if (nblocks <= 0 || !slots || !nbytes || !records || !offsets) return CZW_EINVAL;
+    if (scan_records(nbytes, nblocks, offsets, s) != CZW_OK) return CZW_EHIP;
+    hipLaunchKernelGGL(k_pack_records_f64, dim3(nblocks), dim3(256), 0, s, slots, nbytes, nbytes + nblocks, shuffle,
+                       offsets, records);
+    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
+}
+
+template <int LAYOUT>
+static int launch_decompress_f64(int wt, const uint8_t *rec, uint64_t rb, const uint64_t *offsets, int nblocks,
+                                 int shuffle, double *dst, FieldGeom g, uint32_t *err, const czw_field_t *fields,
+                                 int nf, int groups, uint32_t *queue, hipStream_t s)
+{
+    const void *fn = wt == 1 ? (const void *)k_decompress_f64<1, LAYOUT>
+                   : wt == 2 ? (const void *)k_decompress_f64<2, LAYOUT> : (const void *)k_decompress_f64<3, LAYOUT>;
+    const dim3 grid(std::min(nblocks, groups ? groups : resident_groups(fn, NT))), blk(NT);
+    if (wt == 1) hipLaunchKernelGGL((k_decompress_f64<1, LAYOUT>), grid, blk, 0, s, rec, rb, offsets, nblocks, shuffle, dst, g, err, fields, nf, queue);
+    if (wt == 2) hipLaunchKernelGGL((k_decompress_f64<2, LAYOUT>), grid, blk, 0, s, rec, rb, offsets, nblocks, shuffle, dst, g, err, fields, nf, queue);
+    if (wt == 3) hipLaunchKernelGGL((k_decompress_f64<3, LAYOUT>), grid, blk, 0, s, rec, rb, offsets, nblocks, shuffle, dst, g, err, fields, nf, queue);
+    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
+}
+
+int decompress_f64(int layout, const void *records, size_t rec_bytes, const uint64_t *offsets, int nblocks,
+                   FieldGeom g, int wtype, int shuffle, void *dst, uint32_t *err, const czw_field_t *fields, int nf,
+                   int groups, uint32_t *queue, hipStream_t s)
+{
+    const int wt = wt_index(wtype);
+    if (wt < 0 || nblocks <= 0 || !records || !offsets || !err) return CZW_EINVAL;
+    if (layout != LAYOUT_BATCH && !dst) return CZW_EINVAL;
+    const uint8_t *rec = static_cast<const uint8_t *>(records);
+    double *d = static_cast<double *>(dst);
+    const uint64_t rb = (uint64_t)rec_bytes;
+    switch (layout) {
+    case LAYOUT_BLOCKS:
+        return launch_decompress_f64<LAYOUT_BLOCKS>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, nullptr, 0,
+                                                    groups, queue, s);
+    case LAYOUT_FIELD:
+        return launch_decompress_f64<LAYOUT_FIELD>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, nullptr, 0,
+                                                   groups, queue, s);
+    default:
+        return launch_decompress_f64<LAYOUT_BATCH>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, fields, nf,
+                                                   groups, queue, s);
+    }
+}
+
+}  // namespace czw

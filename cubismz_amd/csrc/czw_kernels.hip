@@ -29,67 +29,16 @@
 
 #include <algorithm>
 
-#include "../../include/czw.h"
-#include "czw_wavelet.hpp"
+#include "czw_common.hpp"
 
 namespace czw {
 
-constexpr int NB3 = 32768;
-constexpr int MASK_BYTES = 4096;
 constexpr int RP = 33;
 constexpr int SP = 32 * 33 + 1;
 constexpr int LDS_WORDS = 32 * SP;
-constexpr int NT = 512;            // threads per block workgroup (8 wave64)
-constexpr int NROW = 1024 / NT;    // level-32 rows (and stream segments) per thread
 constexpr size_t SLOT32 = MASK_BYTES + 4 * (size_t)NB3;
 
-enum { LAYOUT_BLOCKS = 0, LAYOUT_FIELD = 1, LAYOUT_BATCH = 2 };
-constexpr int BATCH_MAX_FIELDS = 128;  // field table entries one launch keeps in LDS
 
-template <int N, typename R>
-CZW_DI void lds_ld(const R *A, int base, int stride, R *r)
-{
-#pragma unroll
-    for (int k = 0; k < N; ++k) r[k] = A[base + k * stride];
-}
-
-template <int N, typename R>
-CZW_DI void lds_st(R *A, int base, int stride, const R *r)
-{
-#pragma unroll
-    for (int k = 0; k < N; ++k) A[base + k * stride] = r[k];
-}
-
-// Wave64 inclusive prefix sum.
-CZW_DI uint32_t wave_incl_scan(uint32_t x, int lane)
-{
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
-    return x;
-}
-
-// Block-wide exclusive scan of one value per thread.  Returns the exclusive
-// prefix; *total receives the sum.  Ends with the wave totals read, so the
-// caller must barrier before wsum is written again.
-CZW_DI uint32_t block_excl_scan(uint32_t v, uint32_t *wsum, uint32_t *total)
-{
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const uint32_t inc = wave_incl_scan(v, lane);
-    if (lane == 63) wsum[w] = inc;
-    __syncthreads();
-    uint32_t base = 0, tot = 0;
-#pragma unroll
-    for (int i = 0; i < NT / 64; ++i) {
-        const uint32_t s = wsum[i];
-        base += (i < w) ? s : 0u;
-        tot += s;
-    }
-    *total = tot;
-    return base + inc - v;
-}
 
 // ---------------------------------------------------------------------------
 // Stream segment g covers stream indices [32g, 32g+32) (g = t + NT*k for the
@@ -165,86 +114,11 @@ CZW_DI void fwt32_yz_f32(float *A, int t)
     __syncthreads();
 }
 
-// Remaining forward passes: level 16 (x, y) and level 8 (y fused, z, x).
-template <int WT>
-CZW_DI void fwt16_8_f32(float *A, int t)
-{
-    float r[16];
-    if (t < 256) {  // level 16 x-pass: row (z, y) in [0,16)^2
-        const int z = t >> 4, y = t & 15;
-        lds_ld<16>(A, z * SP + y * RP, 1, r);
-        row<16, true, WT>(r);
-        lds_st<16>(A, z * SP + y * RP, 1, r);
-    }
-    __syncthreads();
-    if (t < 256) {  // level 16 y-pass, then level 8 y-pass: row (z, x)
-        const int z = t >> 4, x = t & 15;
-        lds_ld<16>(A, z * SP + x, RP, r);
-        row<16, true, WT>(r);
-        if (z < 8 && x < 8) row<8, true, WT>(r);
-        lds_st<16>(A, z * SP + x, RP, r);
-    }
-    __syncthreads();
-    if (t < 64) {  // level 8 z-pass: row (y, x) in [0,8)^2
-        const int y = t >> 3, x = t & 7;
-        lds_ld<8>(A, y * RP + x, SP, r);
-        row<8, true, WT>(r);
-        lds_st<8>(A, y * RP + x, SP, r);
-    }
-    __syncthreads();
-    if (t < 64) {  // level 8 x-pass: row (z, y)
-        const int z = t >> 3, y = t & 7;
-        lds_ld<8>(A, z * SP + y * RP, 1, r);
-        row<8, true, WT>(r);
-        lds_st<8>(A, z * SP + y * RP, 1, r);
-    }
-    __syncthreads();
-}
-
-// Inverse passes of all levels except the final level-32 x-pass, which the
-// caller fuses with the store: level 8 (x, z, y), level 16 (y fused, x, z),
-// level 32 (z fused, y).
-template <int WT>
-CZW_DI void iwt16_8_f32(float *A, int t)
-{
-    {
-        float r[16];
-        if (t < 64) {  // level 8 x-pass
-            const int z = t >> 3, y = t & 7;
-            lds_ld<8>(A, z * SP + y * RP, 1, r);
-            row<8, false, WT>(r);
-            lds_st<8>(A, z * SP + y * RP, 1, r);
-        }
-        __syncthreads();
-        if (t < 64) {  // level 8 z-pass
-            const int y = t >> 3, x = t & 7;
-            lds_ld<8>(A, y * RP + x, SP, r);
-            row<8, false, WT>(r);
-            lds_st<8>(A, y * RP + x, SP, r);
-        }
-        __syncthreads();
-        if (t < 256) {  // level 8 y-pass, then level 16 y-pass
-            const int z = t >> 4, x = t & 15;
-            lds_ld<16>(A, z * SP + x, RP, r);
-            if (z < 8 && x < 8) row<8, false, WT>(r);
-            row<16, false, WT>(r);
-            lds_st<16>(A, z * SP + x, RP, r);
-        }
-        __syncthreads();
-        if (t < 256) {  // level 16 x-pass
-            const int z = t >> 4, y = t & 15;
-            lds_ld<16>(A, z * SP + y * RP, 1, r);
-            row<16, false, WT>(r);
-            lds_st<16>(A, z * SP + y * RP, 1, r);
-        }
-        __syncthreads();
-    }
-}
 
 template <int WT>
 CZW_DI void iwt_most_f32(float *A, int t)
 {
-    iwt16_8_f32<WT>(A, t);
+    iwt16_8<WT, RP, SP>(A, t);
 #pragma unroll
     for (int k = 0; k < NROW; ++k) {  // level 16 z-pass, then level 32 z-pass
         const int q = t + NT * k, y = q >> 5, x = q & 31;
@@ -266,69 +140,7 @@ CZW_DI void iwt_most_f32(float *A, int t)
     __syncthreads();
 }
 
-struct FieldGeom {
-    int nx, ny, nz;
-};
 
-// The block a workgroup works on, resolved from the batch-wide index b: the
-// field it belongs to (source or destination, geometry, threshold) and its
-// index lb inside that field.  Outside LAYOUT_BATCH this is the launch's one
-// field and lb = b.
-struct BlockRef {
-    const float *src;
-    float *dst;
-    FieldGeom g;
-    int lb;
-    float eps;
-};
-
-// Field table of a batch launch, copied into LDS once per workgroup.
-template <int LAYOUT>
-CZW_DI void load_table(czw_field_t *T, const czw_field_t *__restrict__ fields, int nf)
-{
-    if constexpr (LAYOUT == LAYOUT_BATCH) {
-        constexpr int W = sizeof(czw_field_t) / 4;
-        const uint32_t *s = reinterpret_cast<const uint32_t *>(fields);
-        uint32_t *d = reinterpret_cast<uint32_t *>(T);
-        for (int i = threadIdx.x; i < nf * W; i += blockDim.x) d[i] = s[i];
-        __syncthreads();
-    }
-}
-
-template <int LAYOUT>
-CZW_DI BlockRef locate(const czw_field_t *T, int nf, int b, const float *src, float *dst, FieldGeom g, float eps)
-{
-    if constexpr (LAYOUT != LAYOUT_BATCH) {
-        return BlockRef{src, dst, g, b, eps};
-    } else {
-        int lo = 0, hi = nf - 1;  // the last field whose first block is <= b
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (T[mid].block0 <= b) lo = mid;
-            else hi = mid - 1;
-        }
-        const czw_field_t &f = T[lo];
-        return BlockRef{static_cast<const float *>(f.src), static_cast<float *>(f.dst), FieldGeom{f.nx, f.ny, f.nz},
-                        b - f.block0, f.eps};
-    }
-}
-
-// Load/store layout of a block's rows: batch fields are HDF5-ordered fields.
-template <int LAYOUT>
-constexpr int row_layout()
-{
-    return LAYOUT == LAYOUT_BLOCKS ? LAYOUT_BLOCKS : LAYOUT_FIELD;
-}
-
-// Global element offset of block b's row start (x = 0) for the x-row (y, z)
-// of an HDF5-ordered field, and the stride along x.
-CZW_DI size_t field_row(const FieldGeom &g, int b, int y, int z, size_t *xstride)
-{
-    const int bx = g.nx >> 5, by = g.ny >> 5;
-    const int ix = b % bx, iy = (b / bx) % by, iz = b / (bx * by);
-    *xstride = (size_t)g.ny * (size_t)g.nz;
-    return ((size_t)(32 * ix) * g.ny + (size_t)(32 * iy + y)) * g.nz + (size_t)(32 * iz + z);
-}
 
 // ---------------------------------------------------------------------------
 // x-row q (0..1023) of a block: in the block layout q = (z, y) with y
@@ -412,31 +224,6 @@ constexpr int NRUN32 = 1792;  // level-32 stream runs: 7 codes x 16 x 16 runs of
 constexpr int STAGE32_AT = 16 * SP;                      // the z >= 16 half of A
 constexpr int STAGE32 = ((LDS_WORDS - STAGE32_AT - 64) / 4) * 4;  // staged survivors (64 dummy words follow)
 
-// Stream positions [8p, 8p+8) of the [0,16)^3 corner, p < 512 (see
-// for_each_run): two runs of 4 for p < 64 (base 4^3 and level-8 details),
-// one run of 8 otherwise (level-16 details).
-template <typename F>
-CZW_DI void corner_piece(int p, F &&f)
-{
-    if (p < 64) {
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int s = 8 * p + 4 * q;
-            const int code = s >> 6, k = s & 63;
-            const int a0 = 4 * ((code >> 2) & 1) + (k >> 4);
-            const int a1 = 4 * ((code >> 1) & 1) + ((k >> 2) & 3);
-            const int o2 = 4 * (code & 1);
-            f(q, 4, a0 * SP + a1 * RP + o2, 1, a1 + 32 * a0, o2);
-        }
-    } else {
-        const int s = 8 * p;
-        const int code = s >> 9, k = s & 511;
-        const int a0 = 8 * ((code >> 2) & 1) + (k >> 6);
-        const int a1 = 8 * ((code >> 1) & 1) + ((k >> 3) & 7);
-        const int o2 = 8 * (code & 1);
-        f(0, 8, a1 * SP + o2 * RP + a0, RP, a1 + 32 * a0, o2);
-    }
-}
 
 //
 // Block order: workgroup w starts with block w; the next block is w + G
@@ -465,7 +252,7 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
     int b = blockIdx.x;
     float eps_b = eps;
     if (b < nblocks) {
-        const BlockRef br = locate<LAYOUT>(T, nf, b, src, nullptr, g, eps);
+        const BlockRef<float> br = locate<LAYOUT, float>(T, nf, b, src, nullptr, g, eps);
         eps_b = br.eps;
 #pragma unroll
         for (int k = 0; k < NROW; ++k) load_xrow_f32<RL>(br.src, br.g, br.lb, t + NT * k, nx[k]);
@@ -491,7 +278,7 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
         const int bn = min(bnext, nblocks - 1);
         const float eps = eps_b;
         {
-            const BlockRef br = locate<LAYOUT>(T, nf, bn, src, nullptr, g, eps);
+            const BlockRef<float> br = locate<LAYOUT, float>(T, nf, bn, src, nullptr, g, eps);
             eps_b = br.eps;
             load_xrow_f32<RL>(br.src, br.g, br.lb, t, nx[0]);
         }
@@ -590,7 +377,7 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
             }
         }
         {  // second half of the prefetch
-            const BlockRef br = locate<LAYOUT>(T, nf, bn, src, nullptr, g, eps);
+            const BlockRef<float> br = locate<LAYOUT, float>(T, nf, bn, src, nullptr, g, eps);
             load_xrow_f32<RL>(br.src, br.g, br.lb, t + NT, nx[1]);
         }
         __syncthreads();  // stage complete, wsum free again
@@ -600,13 +387,13 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
             for (uint32_t i = t; i < S32 / 4; i += NT) d4[i] = s4[i];
             for (uint32_t i = (S32 & ~3u) + t; i < S32; i += NT) d32[i] = st32[i];
         }
-        fwt16_8_f32<WT>(A, t);
+        fwt16_8<WT, RP, SP>(A, t);
 
         // the [0,16)^3 corner: stream positions [0,4096) in pieces of 8 per
         // thread (base 4^3 always kept); values re-read from A
         uint32_t keep = 0;
         float v[8];
-        corner_piece(t, [&](int q, int h, int addr, int stride, int word, int shift) {
+        corner_piece<RP, SP>(t, [&](int q, int h, int addr, int stride, int word, int shift) {
             uint32_t bits = 0;
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
@@ -672,6 +459,12 @@ __global__ __launch_bounds__(1024) void k_scan_records(const uint32_t *__restric
         run += (uint64_t)nbytes[i] + 4u;
     }
     if (t == 0) offsets[n] = tot;
+}
+
+int scan_records(const uint32_t *nbytes, int n, uint64_t *offsets, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_scan_records, dim3(1), dim3(1024), 0, s, nbytes, n, offsets);
+    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
 
 // Frame each block's payload into the record stream
@@ -807,7 +600,7 @@ CZW_DI bool decode_block_fast_f32(float *A, uint32_t *M, uint32_t *cnt, uint32_t
     if (t < 16 && (M[(t & 3) + 32 * (t >> 2)] & 0xfu) != 0xfu) *flag = 0u;
     // counts in stream order: corner piece t at cnt[t], level-32 run rho at cnt[512 + rho]
     uint32_t keep = 0;
-    corner_piece(t, [&](int q, int h, int, int, int word, int shift) {
+    corner_piece<RP, SP>(t, [&](int q, int h, int, int, int word, int shift) {
         const uint32_t hm = (1u << h) - 1u;
         keep |= ((t < 8) ? hm : ((M[word] >> shift) & hm)) << (q * h);
     });
@@ -848,7 +641,7 @@ CZW_DI bool decode_block_fast_f32(float *A, uint32_t *M, uint32_t *cnt, uint32_t
     const uint32_t dummy = (uint32_t)STAGE32 + (t & 63);
     {  // corner positions: survivor or +0
         uint32_t idx = cnt[t];
-        corner_piece(t, [&](int q, int h, int addr, int stride, int, int) {
+        corner_piece<RP, SP>(t, [&](int q, int h, int addr, int stride, int, int) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 if (i < h) {
@@ -937,7 +730,7 @@ __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict
                 if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_POPCOUNT);
                 continue;
             }
-            iwt16_8_f32<WT>(A, t);
+            iwt16_8<WT, RP, SP>(A, t);
 #pragma unroll
             for (int k = 0; k < NROW; ++k) {  // level 16 z-pass (corner columns), then level 32 z-pass
                 const int q = t + NT * k, y = q >> 5, x = q & 31;
@@ -1046,7 +839,7 @@ __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict
                 lds_st<32>(A, base, 1, rr);
             }
             __syncthreads();
-            const BlockRef br = locate<LAYOUT>(T, nf, b, nullptr, dst, g, 0.0f);
+            const BlockRef<float> br = locate<LAYOUT, float>(T, nf, b, nullptr, dst, g, 0.0f);
             size_t xs;
             float *p0 = br.dst + field_row(br.g, br.lb, 0, 0, &xs);
             const int nz = br.g.nz;
@@ -1064,35 +857,6 @@ __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict
 // ---------------------------------------------------------------------------
 // host-side dispatch
 
-static int wt_index(int wtype)
-{
-    switch (wtype) {
-    case 1: return 1;
-    case 2: return 2;
-    case 0:
-    case 3: return 3;
-    default: return -1;
-    }
-}
-
-// Workgroups that fit on the current device at once for a kernel with the
-// given block size (the persistent kernels use one 139 KB LDS block per CU).
-static int resident_groups(const void *kernel, int threads)
-{
-    static int cus[64];
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (cus[dev] == 0) {
-        int n = 0;
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
-        cus[dev] = n;
-    }
-    int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per <= 0) per = 1;
-    return cus[dev] * per;
-}
-
-static bool zb_ok(int zb) { return zb == 0 || zb == 4 || zb == 8 || zb == 12 || zb == 16; }
 
 template <int LAYOUT>
 static int compress_slots_f32(const float *src, int nblocks, FieldGeom g, float eps, int wtype, int zerobits,
@@ -1116,7 +880,7 @@ static int pack_records(const uint8_t *slots, size_t slot_bytes, const uint32_t 
                         uint8_t *records, uint64_t *offsets, hipStream_t s)
 {
     if (nblocks <= 0 || !slots || !nb || !records || !offsets || (slot_bytes & 3)) return CZW_EINVAL;
-    hipLaunchKernelGGL(k_scan_records, dim3(1), dim3(1024), 0, s, nb, nblocks, offsets);
+    if (scan_records(nb, nblocks, offsets, s) != CZW_OK) return CZW_EHIP;
     hipLaunchKernelGGL(k_pack_records, dim3(nblocks), dim3(256), 0, s, slots, slot_bytes, nb, nblocks, shuffle, offsets,
                        records);
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
@@ -1132,6 +896,18 @@ static int compress_f32(const float *src, int nblocks, FieldGeom g, float eps, i
     const int rc = compress_slots_f32<LAYOUT>(src, nblocks, g, eps, wtype, zerobits, slots, nb, s);
     if (rc != CZW_OK) return rc;
     return pack_records(slots, SLOT32, nb, nblocks, shuffle, static_cast<uint8_t *>(records), offsets, s);
+}
+
+static int compress_f64(int layout, const void *src, int nblocks, FieldGeom g, float eps, int wtype, int zerobits,
+                        int shuffle, void *records, uint64_t *offsets, void *workspace, hipStream_t s)
+{
+    if (!records || !offsets || !workspace || nblocks <= 0) return CZW_EINVAL;
+    uint8_t *slots = static_cast<uint8_t *>(workspace);
+    uint32_t *nb = reinterpret_cast<uint32_t *>(slots + (MASK_BYTES + 8 * (size_t)NB3) * (size_t)nblocks);
+    const int rc = compress_slots_f64(layout, src, nblocks, g, eps, wtype, zerobits, slots, nb, nullptr, 0, 0,
+                                      nullptr, s);
+    if (rc != CZW_OK) return rc;
+    return pack_records_f64(slots, nb, nblocks, shuffle, static_cast<uint8_t *>(records), offsets, s);
 }
 
 template <int LAYOUT>
@@ -1152,11 +928,6 @@ static int decompress_f32(const void *records, size_t rec_bytes, const uint64_t 
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
 
-static bool field_ok(int nx, int ny, int nz)
-{
-    return nx > 0 && ny > 0 && nz > 0 && !(nx & 31) && !(ny & 31) && !(nz & 31) &&
-           (size_t)(nx / 32) * (ny / 32) * (nz / 32) < (size_t)INT32_MAX;
-}
 
 // ---------------------------------------------------------------------------
 // batched mode
@@ -1173,7 +944,8 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, int n, uint32_t v)
 static uint32_t *batch_queue(const czw_batch_t *B)
 {
     const size_t slot = MASK_BYTES + (size_t)B->prec * NB3;
-    return reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(B->workspace) + (slot + 8) * (size_t)B->nblocks);
+    return reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(B->workspace) +
+                                        (slot + 4 * size_words(B->prec)) * (size_t)B->nblocks);
 }
 
 static bool batch_ok(const czw_batch_t *B)
@@ -1185,12 +957,19 @@ static bool batch_ok(const czw_batch_t *B)
 static int batch_compress(const czw_batch_t *B, hipStream_t s)
 {
     if (!batch_ok(B) || !zb_ok(B->zerobits)) return CZW_EINVAL;
-    if (B->prec != 4) return B->prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
+    if (B->prec != 4 && B->prec != 8) return CZW_EINVAL;
     const int wt = wt_index(B->wtype);
     uint8_t *slots = static_cast<uint8_t *>(B->workspace);
-    uint32_t *nb = reinterpret_cast<uint32_t *>(slots + SLOT32 * (size_t)B->nblocks);
+    const size_t slot_bytes = MASK_BYTES + (size_t)B->prec * NB3;
+    uint32_t *nb = reinterpret_cast<uint32_t *>(slots + slot_bytes * (size_t)B->nblocks);
     uint32_t *q = B->queue ? batch_queue(B) : nullptr;
     if (q) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, q, 1, 0u);
+    if (B->prec == 8) {
+        const int rc = compress_slots_f64(LAYOUT_BATCH, nullptr, B->nblocks, FieldGeom{0, 0, 0}, 0.0f, B->wtype,
+                                          B->zerobits, slots, nb, B->fields, B->nfields, B->groups, q, s);
+        if (rc != CZW_OK) return rc;
+        return pack_records_f64(slots, nb, B->nblocks, B->shuffle, static_cast<uint8_t *>(B->records), B->offsets, s);
+    }
     const uint32_t zkeep = B->zerobits ? ~((1u << B->zerobits) - 1u) : 0xffffffffu;
     const void *fn = wt == 1 ? (const void *)k_compress_f32<1, LAYOUT_BATCH>
                    : wt == 2 ? (const void *)k_compress_f32<2, LAYOUT_BATCH> : (const void *)k_compress_f32<3, LAYOUT_BATCH>;
@@ -1208,11 +987,14 @@ static int batch_compress(const czw_batch_t *B, hipStream_t s)
 static int batch_decompress(const czw_batch_t *B, hipStream_t s)
 {
     if (!batch_ok(B) || !B->err) return CZW_EINVAL;
-    if (B->prec != 4) return B->prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
+    if (B->prec != 4 && B->prec != 8) return CZW_EINVAL;
     const int wt = wt_index(B->wtype);
     uint32_t *q = B->queue ? batch_queue(B) + 1 : nullptr;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, B->err, 1, 0u);
     if (q) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, q, 1, 0u);
+    if (B->prec == 8)
+        return decompress_f64(LAYOUT_BATCH, B->records, B->records_bytes, B->offsets, B->nblocks, FieldGeom{0, 0, 0},
+                              B->wtype, B->shuffle, nullptr, B->err, B->fields, B->nfields, B->groups, q, s);
     const uint8_t *rec = static_cast<const uint8_t *>(B->records);
     const void *fn = wt == 1 ? (const void *)k_decompress_f32<1, LAYOUT_BATCH>
                    : wt == 2 ? (const void *)k_decompress_f32<2, LAYOUT_BATCH> : (const void *)k_decompress_f32<3, LAYOUT_BATCH>;
@@ -1239,7 +1021,7 @@ size_t czw_compress_workspace_bytes(int nblocks, int prec)
 {
     if (nblocks <= 0 || (prec != 4 && prec != 8)) return 0;
     const size_t slot = MASK_BYTES + (size_t)prec * NB3;
-    return slot * (size_t)nblocks + 8 * (size_t)nblocks;
+    return (slot + 4 * size_words(prec)) * (size_t)nblocks;
 }
 
 size_t czw_records_bound(int nblocks, int prec)
@@ -1257,7 +1039,9 @@ int czw_compress_field(const void *field, int nx, int ny, int nz, int prec, floa
     if (prec == 4)
         return compress_f32<LAYOUT_FIELD>(static_cast<const float *>(field), nb, g, eps, wtype, zerobits, shuffle,
                                           records, offsets, workspace, stream);
-    return prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
+    if (prec == 8) return compress_f64(LAYOUT_FIELD, field, nb, g, eps, wtype, zerobits, shuffle, records, offsets,
+                                       workspace, stream);
+    return CZW_EINVAL;
 }
 
 int czw_compress_field_slots(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype,
@@ -1269,12 +1053,19 @@ int czw_compress_field_slots(const void *field, int nx, int ny, int nz, int prec
     if (prec == 4)
         return compress_slots_f32<LAYOUT_FIELD>(static_cast<const float *>(field), nb, g, eps, wtype, zerobits,
                                                 static_cast<uint8_t *>(slots), nbytes, stream);
-    return prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
+    if (prec == 8)
+        return compress_slots_f64(LAYOUT_FIELD, field, nb, g, eps, wtype, zerobits, static_cast<uint8_t *>(slots),
+                                  nbytes, nullptr, 0, 0, nullptr, stream);
+    return CZW_EINVAL;
 }
 
 int czw_pack_records(const void *slots, size_t slot_bytes, const uint32_t *nbytes, int nblocks, int shuffle,
                      void *records, uint64_t *offsets, hipStream_t stream)
 {
+    if (slot_bytes == MASK_BYTES + 8 * (size_t)NB3)
+        return pack_records_f64(static_cast<const uint8_t *>(slots), nbytes, nblocks, shuffle,
+                                static_cast<uint8_t *>(records), offsets, stream);
+    if (slot_bytes != SLOT32) return CZW_EINVAL;
     return pack_records(static_cast<const uint8_t *>(slots), slot_bytes, nbytes, nblocks, shuffle,
                         static_cast<uint8_t *>(records), offsets, stream);
 }
@@ -1288,7 +1079,9 @@ int czw_compress_blocks(const void *blocks, int nblocks, int prec, float eps, in
     if (prec == 4)
         return compress_f32<LAYOUT_BLOCKS>(static_cast<const float *>(blocks), nblocks, g, eps, wtype, zerobits,
                                            shuffle, records, offsets, workspace, stream);
-    return prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
+    if (prec == 8) return compress_f64(LAYOUT_BLOCKS, blocks, nblocks, g, eps, wtype, zerobits, shuffle, records,
+                                       offsets, workspace, stream);
+    return CZW_EINVAL;
 }
 
 int czw_decompress_field(const void *records, size_t records_bytes, const uint64_t *offsets, int nx, int ny, int nz,
@@ -1300,7 +1093,10 @@ int czw_decompress_field(const void *records, size_t records_bytes, const uint64
     if (prec == 4)
         return decompress_f32<LAYOUT_FIELD>(records, records_bytes, offsets, nb, g, wtype, shuffle,
                                             static_cast<float *>(field), err, stream);
-    return prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
+    if (prec == 8)
+        return decompress_f64(LAYOUT_FIELD, records, records_bytes, offsets, nb, g, wtype, shuffle, field, err,
+                              nullptr, 0, 0, nullptr, stream);
+    return CZW_EINVAL;
 }
 
 int czw_decompress_blocks(const void *records, size_t records_bytes, const uint64_t *offsets, int nblocks, int prec,
@@ -1310,7 +1106,10 @@ int czw_decompress_blocks(const void *records, size_t records_bytes, const uint6
     if (prec == 4)
         return decompress_f32<LAYOUT_BLOCKS>(records, records_bytes, offsets, nblocks, g, wtype, shuffle,
                                              static_cast<float *>(blocks), err, stream);
-    return prec == 8 ? CZW_EUNSUPPORTED : CZW_EINVAL;
+    if (prec == 8)
+        return decompress_f64(LAYOUT_BLOCKS, records, records_bytes, offsets, nblocks, g, wtype, shuffle, blocks,
+                              err, nullptr, 0, 0, nullptr, stream);
+    return CZW_EINVAL;
 }
 
 int czw_batch_prepare(czw_field_t *fields, int nfields, int *nblocks)
