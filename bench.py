@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-mode leg (64 uneven fields)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg (8 x 128^3)")
+    ap.add_argument("--no-f64", action="store_true", help="skip the binary64 scale-up leg (1024^3 f64)")
+    ap.add_argument("--f64-size", type=int, default=1024)
     return ap.parse_args()
 
 
@@ -120,6 +122,103 @@ def batch_leg(a, torch, czw, fields):
             "speedup_vs_plain": round(t_plain / t_graph, 3),
             "autotuned": {"compress_ms": round(ms_c, 4), "decompress_ms": round(ms_d, 4), **shape},
             "matches_per_field": same}
+
+
+def f64_leg(a, torch, czw, fields):
+    """1024^3 float64 field of the bubble type, 1600 bubbles (8 octants of 512^3
+    with 200 bubbles each, seeds 1..8), eps 1e-6 and 1e-8: compress + decompress
+    on the GPU, records and decoded values checked against the oracle on the
+    first z-slab of blocks."""
+    import numpy as np
+
+    n = a.f64_size
+    h = n // 2
+    shape = (n, n, n)
+    print(f"[bench] f64 leg: generating {shape} field", file=sys.stderr, flush=True)
+    field = torch.empty(shape, dtype=torch.float64, device="cuda")
+    for o in range(8):
+        ox, oy, oz = (o & 1) * h, ((o >> 1) & 1) * h, (o >> 2) * h
+        field[ox:ox + h, oy:oy + h, oz:oz + h] = fields.bubble_field((h, h, h), seed=1 + o, nbubbles=200,
+                                                                     dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    nblocks = (n // 32) ** 3
+    slot = czw.slot_bytes(8)
+    slots = torch.empty(nblocks * slot, dtype=torch.uint8, device="cuda")
+    nbytes = torch.empty(9 * nblocks, dtype=torch.int32, device="cuda")
+    records = torch.empty(czw.records_bound(nblocks, 8), dtype=torch.uint8, device="cuda")
+    offsets = torch.empty(nblocks + 1, dtype=torch.int64, device="cuda")
+    out = torch.empty_like(field)
+    err = torch.zeros(1, dtype=torch.int32, device="cuda")
+    field_bytes = field.numel() * 8
+    res = {"workload": f"{n}^3 f64 bubble field (1600 bubbles), {nblocks} blocks", "wtype": a.wtype, "eps": {}}
+    steps = max(3, a.steps // 4)
+    for eps in (1e-6, 1e-8):
+        def step(ev=None):
+            if ev:
+                ev[0].record()
+            czw.compress_field_slots(field, eps, a.wtype, 0, slots, nbytes)
+            if ev:
+                ev[1].record()
+            czw.pack_records(slots, slot, nbytes, nblocks, 0, records, offsets)
+            if ev:
+                ev[2].record()
+            czw.decompress_field(records, offsets, shape, torch.float64, a.wtype, 0, out=out, err=err, check=False)
+            if ev:
+                ev[3].record()
+
+        step()
+        torch.cuda.synchronize()
+        if int(err.item()):
+            raise SystemExit("f64 decoder reported corrupt records")
+        evs = [_events(torch, 4) for _ in range(steps)]
+        for k in range(steps):
+            step(evs[k])
+        torch.cuda.synchronize()
+        tc = sum(e[0].elapsed_time(e[1]) for e in evs) / steps * 1e-3
+        tp = sum(e[1].elapsed_time(e[2]) for e in evs) / steps * 1e-3
+        td = sum(e[2].elapsed_time(e[3]) for e in evs) / steps * 1e-3
+        tt = sum(e[0].elapsed_time(e[3]) for e in evs) / steps * 1e-3
+        nb = nbytes[:nblocks].cpu().numpy().astype(np.int64)
+        payload = int(nb.sum())
+        alg_c = field_bytes + payload
+        alg_d = field_bytes + payload + 4 * nblocks
+        r = {"GBps": round(field_bytes / tt / 1e9, 3), "ms_per_step": round(tt * 1e3, 4),
+             "stages_ms": {"compress": round(tc * 1e3, 4), "pack": round(tp * 1e3, 4), "decompress": round(td * 1e3, 4)},
+             "compress_kernel_GBps_alg": round(alg_c / tc / 1e9, 2),
+             "decompress_kernel_GBps_alg": round(alg_d / td / 1e9, 2),
+             "step_roofline_frac": round((alg_c + alg_d) / tt / HBM_PEAK, 4),
+             "survivor_fraction": round(float(((nb - 4096) // 8).sum()) / field.numel(), 6),
+             "ratio_vs_records": round(field_bytes / float(offsets[-1].item()), 3)}
+        if not a.no_cpu and eps == 1e-6:
+            from oracle import oracle as O
+
+            # the first z-slab of blocks (z in [0, 32)) is the sub-field
+            # [n][n][32], whose blocks are numbered like the field's first n^2/1024
+            b1 = (n // 32) ** 2
+            fh = np.ascontiguousarray(field[:, :, :32].cpu().numpy())
+            t0 = time.perf_counter()
+            oslots, onb = O.compress_field(fh, eps, a.wtype, 0, False)
+            t1 = time.perf_counter()
+            ofield = O.decompress_field(oslots, onb, fh.shape, a.wtype, "f64", False)
+            t2 = time.perf_counter()
+            off = offsets[: b1 + 1].cpu().numpy()
+            recs = records[: int(off[-1])].cpu().numpy()
+            mism = 0
+            for b in range(b1):
+                o = int(off[b])
+                if int(recs[o:o + 4].view(np.int32)[0]) != int(onb[b]) or \
+                        not np.array_equal(recs[o + 4:o + 4 + int(onb[b])], oslots[b, :onb[b]]):
+                    mism += 1
+            dec_ok = bool(np.array_equal(out[:, :, :32].cpu().numpy().view(np.uint64),
+                                         ofield[:, :, :32].view(np.uint64)))
+            r["parity"] = {"blocks_checked": b1, "records_bitexact": mism == 0, "blocks_differing": mism,
+                           "decoded_bitexact": dec_ok}
+            r["cpu_oracle_GBps"] = round(b1 * 32768 * 8 / (t2 - t0) / 1e9, 4)
+            del ofield, oslots
+        res["eps"][f"{eps:g}"] = r
+    del field, slots, records, out
+    torch.cuda.empty_cache()
+    return res
 
 
 def adjoint_leg(a, torch, fields):
@@ -253,6 +352,7 @@ def main():
         other[f"wtype{w}"] = {"GBps": round(field_bytes / tw / 1e9, 3), "ms_per_step": round(tw * 1e3, 4)}
     batched = None if a.no_batch else batch_leg(a, torch, czw, fields)
     adjoint = None if a.no_adjoint else adjoint_leg(a, torch, fields)
+    f64 = None if a.no_f64 else f64_leg(a, torch, czw, fields)
     # restore the headline wavelet's records for the checks below
     step()
     torch.cuda.synchronize()
@@ -327,6 +427,7 @@ def main():
         "other_wtypes": other,
         "batched": batched,
         "adjoint": adjoint,
+        "f64_scaleup": f64,
         "compression": {"payload_bytes": payload, "survivor_fraction": round(rho, 5),
                         "ratio_vs_records": round(field_bytes / float(off[-1]), 3),
                         "linf": err_inf, "psnr_db": round(float(psnr), 4)},
