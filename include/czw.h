@@ -87,10 +87,11 @@ int czw_compress_field(const void *field, int nx, int ny, int nz, int prec, floa
 /* The two stages of czw_compress_field, for callers that time the stages
  * apart or keep fixed-size per-block slots.  Stage 1 writes block b's mask and
  * survivors (zerobits applied, not yet shuffled) into slots +
- * b*czw_slot_bytes(prec) in an internal split layout, and sets nbytes[b] to
- * the payload size and nbytes[nblocks+b] to an internal split count (nbytes
- * has 2*nblocks entries).  Stage 2 frames the payloads into the contiguous
- * record stream, applying the byte shuffle when shuffle != 0. */
+ * b*czw_slot_bytes(prec) in an internal region layout, and sets nbytes[b] to
+ * the payload size followed by internal region counts: nbytes has 2*nblocks
+ * entries for prec 4 and 9*nblocks for prec 8.  Stage 2 frames the payloads
+ * into the contiguous record stream, applying the byte shuffle when
+ * shuffle != 0; slot_bytes must be czw_slot_bytes(prec). */
 size_t czw_slot_bytes(int prec);
 int czw_compress_field_slots(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype,
                              int zerobits, void *slots, uint32_t *nbytes, hipStream_t stream);
