@@ -51,6 +51,12 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg (8 x 128^3)")
     ap.add_argument("--no-f64", action="store_true", help="skip the binary64 scale-up leg (1024^3 f64)")
     ap.add_argument("--f64-size", type=int, default=1024)
+    ap.add_argument("--no-cz", action="store_true", help="skip the end-to-end .cz (host zlib) measurement")
+    ap.add_argument("--field-cache", default="",
+                    help="raw file for the headline field: loaded when present, written after generation otherwise "
+                         "(lets profiler runs skip the generator's kernels)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run 5 device copies of a 512 MiB tensor (calibrates HBM byte counters) and exit")
     return ap.parse_args()
 
 
@@ -215,6 +221,15 @@ def f64_leg(a, torch, czw, fields):
                            "decoded_bitexact": dec_ok}
             r["cpu_oracle_GBps"] = round(b1 * 32768 * 8 / (t2 - t0) / 1e9, 4)
             del ofield, oslots
+        if eps == 1e-6 and not a.no_cz:
+            # end to end to .cz bytes: host zlib (OpenMP) overlapped slab by slab with the device
+            from cubismz_amd import czfile
+
+            threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+            fb, tm = czfile.compress_device_field(None, field, eps, a.wtype, nthreads=threads, overlap=True)
+            r["cz_file_end_to_end"] = {"file_bytes": fb, "ratio": round(field_bytes / fb, 3), "threads": threads,
+                                       "GBps": round(field_bytes / tm["wall"] / 1e9, 3),
+                                       "seconds": {k: round(v, 4) for k, v in tm.items()}}
         res["eps"][f"{eps:g}"] = r
     del field, slots, records, out
     torch.cuda.empty_cache()
@@ -280,12 +295,26 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     dev = torch.device("cuda:0")
+    if a.copy_probe:
+        x = torch.ones(1 << 27, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": x.numel() * 4, "copies": 5}))
+        return
     n = a.size
     shape = (n, n, n)
     prec = 4
     nblocks = (n // 32) ** 3
-    print(f"[bench] generating {shape} bubble field", file=sys.stderr, flush=True)
-    field = fields.bubble_field(shape, seed=1, nbubbles=a.bubbles, dtype=torch.float32, device=dev)
+    if a.field_cache and os.path.exists(a.field_cache):
+        print(f"[bench] loading {shape} field from {a.field_cache}", file=sys.stderr, flush=True)
+        field = torch.from_numpy(np.fromfile(a.field_cache, np.float32).reshape(shape)).to(dev)
+    else:
+        print(f"[bench] generating {shape} bubble field", file=sys.stderr, flush=True)
+        field = fields.bubble_field(shape, seed=1, nbubbles=a.bubbles, dtype=torch.float32, device=dev)
+        if a.field_cache:
+            field.cpu().numpy().tofile(a.field_cache)
     torch.cuda.synchronize()
 
     slot = czw.slot_bytes(prec)

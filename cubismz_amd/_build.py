@@ -47,11 +47,16 @@ TOOLS = {"hdf2cz": 1, "cz2hdf": 2, "cz2diff": 3}
 
 
 def build_host():
-    """libczhost.so: .cz container writer/reader (host C++, zlib, OpenMP)."""
+    """libczhost.so: .cz container writer/reader (host C++, zlib, OpenMP) and the
+    device-field writer that overlaps zlib with the device compressor (links
+    libczw.so and the HIP runtime)."""
     os.makedirs(LIBDIR, exist_ok=True)
     out = os.path.join(LIBDIR, "libczhost.so")
-    _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-Wall",
-          os.path.join(CSRC, "cz_file.cpp"), "-o", out + ".tmp", "-lz"])
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-Wall", "-D__HIP_PLATFORM_AMD__",
+          "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(rocm, "include"),
+          os.path.join(CSRC, "cz_file.cpp"), os.path.join(CSRC, "cz_pipeline.cpp"), "-o", out + ".tmp",
+          "-L" + LIBDIR, "-lczw", "-lz", "-L" + os.path.join(rocm, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -32,7 +32,7 @@ def _lib():
     if not getattr(L, "_batch_typed", False):
         vp, i = C.c_void_p, C.c_int
         L.czw_batch_prepare.restype = i
-        L.czw_batch_prepare.argtypes = [vp, i, C.POINTER(C.c_int)]
+        L.czw_batch_prepare.argtypes = [vp, i, i, C.POINTER(C.c_int)]
         L.czw_batch_workspace_bytes.restype = C.c_size_t
         L.czw_batch_workspace_bytes.argtypes = [i, i]
         for n in ("czw_batch_compress", "czw_batch_decompress"):
@@ -50,7 +50,7 @@ def _lib():
     return L
 
 
-def prepare_table(shapes, eps):
+def prepare_table(shapes, eps, prec=4):
     """Host table for fields of the given (nx, ny, nz) shapes: block0/nblocks filled."""
     if not 0 < len(shapes) <= MAX_FIELDS or len(eps) != len(shapes):
         raise czw.CZWError(f"a batch holds 1..{MAX_FIELDS} fields, one eps each")
@@ -59,7 +59,7 @@ def prepare_table(shapes, eps):
         t[k]["nx"], t[k]["ny"], t[k]["nz"] = s
         t[k]["eps"] = np.float32(e)
     total = C.c_int(0)
-    rc = _lib().czw_batch_prepare(C.c_void_p(t.ctypes.data), len(t), C.byref(total))
+    rc = _lib().czw_batch_prepare(C.c_void_p(t.ctypes.data), len(t), int(prec), C.byref(total))
     if rc != 0:
         raise czw.CZWError("czw_batch_prepare: invalid field table (extents must be multiples of 32)")
     return t, int(total.value)
@@ -105,7 +105,7 @@ class Batch:
             raise czw.CZWError("batch fields must be contiguous device tensors [nx, ny, nz] of one dtype, "
                                "float32 or float64")
         self.fields = list(fields)
-        self.table, self.nblocks = prepare_table([tuple(f.shape) for f in fields], eps)
+        self.table, self.nblocks = prepare_table([tuple(f.shape) for f in fields], eps, czw._prec(dtype, torch))
         dev = self.fields[0].device
         self.outputs = list(outputs) if outputs is not None else [torch.empty_like(f) for f in fields]
         for k, f in enumerate(self.fields):

@@ -126,13 +126,13 @@ bool inflate_chunk(const uint8_t *in, size_t n, std::vector<uint8_t> &out)
     return true;
 }
 
-int build_cz(const Params &p, const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads,
-             std::vector<uint8_t> &file)
+int encode_chunks(const Params &p, const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads,
+                  std::vector<std::vector<uint8_t>> &chunks)
 {
-    if (nblocks <= 0 || p.bpd[0] * p.bpd[1] * p.bpd[2] != nblocks) return -1;
+    if (nblocks <= 0) return -1;
     const int E = entries_per_chunk(p.prec);
     const int nchunks = (nblocks + E - 1) / E;
-    std::vector<std::vector<uint8_t>> chunks(nchunks);
+    chunks.assign(nchunks, std::vector<uint8_t>());
     int bad = 0;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -148,8 +148,15 @@ int build_cz(const Params &p, const uint8_t *records, const uint64_t *offsets, i
         }
     }
     (void)nthreads;
-    if (bad) return -2;
+    return bad ? -2 : 0;
+}
 
+int assemble_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks,
+                std::vector<uint8_t> &file)
+{
+    const int E = entries_per_chunk(p.prec);
+    const int nchunks = (int)chunks.size();
+    if (nblocks <= 0 || p.bpd[0] * p.bpd[1] * p.bpd[2] != nblocks || nchunks != (nblocks + E - 1) / E) return -1;
     const std::string ocean(OCEAN_TITLE), luttitle(LUT_TITLE);
     const std::string header = make_header(p);
     size_t written = 0;
@@ -183,6 +190,16 @@ int build_cz(const Params &p, const uint8_t *records, const uint64_t *offsets, i
     const HeaderLUT hl = {(uint64_t)aggregate, nchunks};
     put(&hl, sizeof(hl));
     return 0;
+}
+
+int build_cz(const Params &p, const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads,
+             std::vector<uint8_t> &file)
+{
+    if (nblocks <= 0 || p.bpd[0] * p.bpd[1] * p.bpd[2] != nblocks) return -1;
+    std::vector<std::vector<uint8_t>> chunks;
+    const int rc = encode_chunks(p, records, offsets, nblocks, nthreads, chunks);
+    if (rc) return rc;
+    return assemble_cz(p, chunks, nblocks, file);
 }
 
 static bool parse_kv(const std::string &line, const char *key, std::string &val)

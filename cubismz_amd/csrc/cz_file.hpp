@@ -77,6 +77,13 @@ bool inflate_chunk(const uint8_t *in, size_t n, std::vector<uint8_t> &out);
 // complete single-subdomain .cz image.  records/offsets as in czw.h.
 int build_cz(const Params &p, const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads,
              std::vector<uint8_t> &file);
+// The two halves of build_cz: the chunk stage (zlib per chunk of ENTRIES
+// records, OpenMP over chunks; offsets relative to records) and the file
+// image around the finished chunks.
+int encode_chunks(const Params &p, const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads,
+                  std::vector<std::vector<uint8_t>> &chunks);
+int assemble_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks,
+                std::vector<uint8_t> &file);
 
 struct FileInfo {
     Params p;

@@ -22,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/cz_host.h"
 #include "../../include/czw.h"
 #include "cz_file.hpp"
 
@@ -288,9 +289,10 @@ int tool_main(int argc, char **argv)
     p.threshold = threshold;
     p.encoder = a.str("-encoder", "zlib") == "none" ? cz::ENC_NONE : cz::ENC_ZLIB;
     const int nb = p.bpd[0] * p.bpd[1] * p.bpd[2];
-    std::vector<uint8_t> rec;
-    std::vector<uint64_t> off(nb + 1);
+    size_t file_size = 0;
     if (wtype < 0) {  // raw blocks [z][y][x]
+        std::vector<uint8_t> rec;
+        std::vector<uint64_t> off(nb + 1);
         const int rb = 4 + 32768 * f.prec;
         rec.resize((size_t)nb * rb);
         for (int b = 0; b < nb; ++b) {
@@ -307,36 +309,29 @@ int tool_main(int argc, char **argv)
                     }
         }
         off[nb] = rec.size();
-    } else {
-        void *dfield, *drec, *dws;
-        uint64_t *doff;
-        const size_t rb = czw_records_bound(nb, f.prec), wb = czw_compress_workspace_bytes(nb, f.prec);
-        HIPCHECK(hipMalloc(&dfield, f.data.size()));
-        HIPCHECK(hipMalloc(&drec, rb));
-        HIPCHECK(hipMalloc(&dws, wb));
-        HIPCHECK(hipMalloc(&doff, (nb + 1) * 8));
-        HIPCHECK(hipMemcpy(dfield, f.data.data(), f.data.size(), hipMemcpyHostToDevice));
-        const int rc = czw_compress_field(dfield, f.n[0], f.n[1], f.n[2], f.prec, (float)threshold, wtype, zerobits,
-                                          shuffle, drec, doff, dws, 0);
-        if (rc != CZW_OK) {
-            std::fprintf(stderr, "czw_compress_field failed (%d)\n", rc);
+        std::vector<uint8_t> file;
+        if (cz::build_cz(p, rec.data(), off.data(), nb, 0, file) || !cz::write_file(path, file)) {
+            std::fprintf(stderr, "hdf2cz: writing %s failed\n", path.c_str());
             return 1;
         }
-        HIPCHECK(hipMemcpy(off.data(), doff, (nb + 1) * 8, hipMemcpyDeviceToHost));
-        rec.resize(off[nb]);
-        HIPCHECK(hipMemcpy(rec.data(), drec, rec.size(), hipMemcpyDeviceToHost));
+        file_size = file.size();
+    } else {
+        // device compressor, zlib on host threads overlapped slab by slab
+        void *dfield;
+        HIPCHECK(hipMalloc(&dfield, f.data.size()));
+        HIPCHECK(hipMemcpy(dfield, f.data.data(), f.data.size(), hipMemcpyHostToDevice));
+        uint64_t fb = 0;
+        const int rc = czh_compress_device_field(path.c_str(), dfield, f.n[0], f.n[1], f.n[2], f.prec, (float)threshold,
+                                                 wtype, zerobits, shuffle, p.encoder, 0, 0, 1, 0, &fb, nullptr);
         (void)hipFree(dfield);
-        (void)hipFree(drec);
-        (void)hipFree(dws);
-        (void)hipFree(doff);
-    }
-    std::vector<uint8_t> file;
-    if (cz::build_cz(p, rec.data(), off.data(), nb, 0, file) || !cz::write_file(path, file)) {
-        std::fprintf(stderr, "hdf2cz: writing %s failed\n", path.c_str());
-        return 1;
+        if (rc != CZH_OK) {
+            std::fprintf(stderr, "hdf2cz: compressing to %s failed (%d)\n", path.c_str(), rc);
+            return 1;
+        }
+        file_size = fb;
     }
     const double t1 = now();
-    const size_t written = file.size() - (cz::make_header(p).size() + nb * sizeof(cz::BlockMetadata) +
+    const size_t written = file_size - (cz::make_header(p).size() + nb * sizeof(cz::BlockMetadata) +
                                           std::strlen(cz::LUT_TITLE) + sizeof(cz::HeaderLUT) + 56);
     std::printf("Channel %d: %.2f kB, wavelet-threshold: %.1e, compr. rate: %.2f\n", 0, written / 1024., threshold,
                 32768.0 * f.prec * nb / (float)written);

@@ -18,7 +18,11 @@ constexpr int MASK_BYTES = 4096;
 constexpr int NT = 512;            // threads per block workgroup (8 wave64)
 constexpr int NROW = 1024 / NT;    // level-32 rows (and stream segments) per thread
 
-enum { LAYOUT_BLOCKS = 0, LAYOUT_FIELD = 1, LAYOUT_BATCH = 2 };
+// LAYOUT_FIELD addresses a block's rows with buffer instructions: 32-bit
+// byte offsets from the block origin, the x step in a scalar register (no
+// per-element address arithmetic).  Fields whose 32 x-planes exceed that
+// range use LAYOUT_FIELD_WIDE (64-bit addresses).
+enum { LAYOUT_BLOCKS = 0, LAYOUT_FIELD = 1, LAYOUT_BATCH = 2, LAYOUT_FIELD_WIDE = 3 };
 constexpr int BATCH_MAX_FIELDS = 128;  // field table entries one launch keeps in LDS
 
 template <int N, typename R>
@@ -66,14 +70,18 @@ CZW_DI uint32_t block_excl_scan(uint32_t v, uint32_t *wsum, uint32_t *total)
     return base + inc - v;
 }
 
+// An HDF5-ordered field [nx][ny][nz]; a launch's block b is the field's
+// block b0 + b (b0 > 0 when a launch covers a range of the field's blocks).
 struct FieldGeom {
     int nx, ny, nz;
+    int b0 = 0;
 };
 
 // Global element offset of block b's row start (x = 0) for the x-row (y, z)
 // of an HDF5-ordered field, and the stride along x.
 CZW_DI size_t field_row(const FieldGeom &g, int b, int y, int z, size_t *xstride)
 {
+    b += g.b0;
     const int bx = g.nx >> 5, by = g.ny >> 5;
     const int ix = b % bx, iy = (b / bx) % by, iz = b / (bx * by);
     *xstride = (size_t)g.ny * (size_t)g.nz;
@@ -128,8 +136,34 @@ CZW_DI BlockRef<R> locate(const czw_field_t *T, int nf, int b, const R *src, R *
 template <int LAYOUT>
 constexpr int row_layout()
 {
-    return LAYOUT == LAYOUT_BLOCKS ? LAYOUT_BLOCKS : LAYOUT_FIELD;
+    return LAYOUT == LAYOUT_BLOCKS ? LAYOUT_BLOCKS : (LAYOUT == LAYOUT_FIELD_WIDE ? LAYOUT_FIELD_WIDE : LAYOUT_FIELD);
 }
+
+// Whether every offset inside a block of an [nx][ny][nz] field with Real of
+// prec bytes fits the buffer range used by LAYOUT_FIELD.
+inline bool field_fits_buffer(int ny, int nz, int prec)
+{
+    return 32ull * (unsigned long long)ny * (unsigned long long)nz * (unsigned long long)prec < (1ull << 31);
+}
+
+// A wave-uniform pointer as scalar registers.
+template <typename T>
+CZW_DI T *uniform_ptr(T *p)
+{
+    const unsigned long long u = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return reinterpret_cast<T *>(((unsigned long long)hi << 32) | lo);
+}
+
+// Raw buffer resource over [base, base + 2 GiB) (stride 0, no swizzle).
+CZW_DI __amdgpu_buffer_rsrc_t buffer_rsrc(const void *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(uniform_ptr(base)), 0, 0x7fffffff, 0x00020000);
+}
+
+typedef unsigned czw_u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned czw_u32x4 __attribute__((ext_vector_type(4)));
 
 // Remaining forward passes on the [0,16)^3 corner of a block held in LDS at
 // z*SP_ + y*RP_ + x: level 16 (x, y) and level 8 (y fused, z, x).

@@ -61,6 +61,15 @@ CZW_DI void load_xrow_f64(const double *__restrict__ src, const FieldGeom &g, in
             r[2 * i] = v.x;
             r[2 * i + 1] = v.y;
         }
+    } else if constexpr (RL == LAYOUT_FIELD) {
+        size_t xs;
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(src + field_row(g, b, 0, 0, &xs));
+        const int vo = ((q >> 5) * g.nz + (q & 31)) * 8, so = (int)xs * 8;
+#pragma unroll
+        for (int x = 0; x < 32; ++x) {
+            const czw_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, vo, x * so, 0);
+            r[x] = __longlong_as_double((long long)(((unsigned long long)v.y << 32) | v.x));
+        }
     } else {
         size_t xs;
         const double *p = src + field_row(g, b, q >> 5, q & 31, &xs);
@@ -76,6 +85,16 @@ CZW_DI void store_xrow_f64(double *__restrict__ dst, const FieldGeom &g, int b, 
         double2 *p = reinterpret_cast<double2 *>(dst + (size_t)b * NB3 + (size_t)q * 32);
 #pragma unroll
         for (int i = 0; i < 16; ++i) p[i] = make_double2(r[2 * i], r[2 * i + 1]);
+    } else if constexpr (RL == LAYOUT_FIELD) {
+        size_t xs;
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(dst + field_row(g, b, 0, 0, &xs));
+        const int vo = ((q >> 5) * g.nz + (q & 31)) * 8, so = (int)xs * 8;
+#pragma unroll
+        for (int x = 0; x < 32; ++x) {
+            const unsigned long long u = (unsigned long long)__double_as_longlong(r[x]);
+            const czw_u32x2 v = {(unsigned)u, (unsigned)(u >> 32)};
+            __builtin_amdgcn_raw_buffer_store_b64(v, rs, vo, x * so, 0);
+        }
     } else {
         size_t xs;
         double *p = dst + field_row(g, b, q >> 5, q & 31, &xs);
@@ -571,6 +590,9 @@ int compress_slots_f64(int layout, const void *src, int nblocks, FieldGeom g, fl
     case LAYOUT_FIELD:
         return launch_compress_f64<LAYOUT_FIELD>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, nullptr, 0,
                                                  groups, queue, s);
+    case LAYOUT_FIELD_WIDE:
+        return launch_compress_f64<LAYOUT_FIELD_WIDE>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, nullptr, 0,
+                                                      groups, queue, s);
     default:
         return launch_compress_f64<LAYOUT_BATCH>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, fields, nf,
                                                  groups, queue, s);
@@ -618,6 +640,9 @@ int decompress_f64(int layout, const void *records, size_t rec_bytes, const uint
     case LAYOUT_FIELD:
         return launch_decompress_f64<LAYOUT_FIELD>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, nullptr, 0,
                                                    groups, queue, s);
+    case LAYOUT_FIELD_WIDE:
+        return launch_decompress_f64<LAYOUT_FIELD_WIDE>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, nullptr,
+                                                        0, groups, queue, s);
     default:
         return launch_decompress_f64<LAYOUT_BATCH>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, fields, nf,
                                                    groups, queue, s);

@@ -88,54 +88,52 @@ CZW_DI void for_each_run(int g, F &&f)
 }
 
 // ---------------------------------------------------------------------------
-// Forward level-32 y- and z-passes (the x-pass runs on the rows as loaded),
-// with the level-16 z-pass fused into the level-32 z-pass.
-template <int WT>
-CZW_DI void fwt32_yz_f32(float *A, int t)
+// One level-32 pass over this thread's NROW rows (row k starts at base[k],
+// elements `stride` apart): every row is loaded before the first one is
+// written back, so the loads of the second row overlap the arithmetic of the
+// first (the compiler cannot hoist them past stores to the same array).
+template <bool FWD, int WT>
+CZW_DI void pass_rows_f32(float *A, const int (&base)[NROW], int stride)
 {
+    float r[NROW][32];
 #pragma unroll
-    for (int k = 0; k < NROW; ++k) {  // y-pass: row (z, x)
-        const int q = t + NT * k, z = q >> 5, x = q & 31;
-        float r[32];
-        lds_ld<32>(A, z * SP + x, RP, r);
-        row<32, true, WT>(r);
-        lds_st<32>(A, z * SP + x, RP, r);
-    }
-    __syncthreads();
+    for (int k = 0; k < NROW; ++k) lds_ld<32>(A, base[k], stride, r[k]);
 #pragma unroll
-    for (int k = 0; k < NROW; ++k) {  // z-pass: row (y, x); then level 16 on y, x < 16
-        const int q = t + NT * k, y = q >> 5, x = q & 31;
-        float r[32];
-        lds_ld<32>(A, y * RP + x, SP, r);
-        row<32, true, WT>(r);
-        if (y < 16 && x < 16) row<16, true, WT>(r);
-        lds_st<32>(A, y * RP + x, SP, r);
+    for (int k = 0; k < NROW; ++k) {
+        row<32, FWD, WT>(r[k]);
+        lds_st<32>(A, base[k], stride, r[k]);
     }
-    __syncthreads();
 }
 
+// Rows of the level-32 passes owned by thread t: y-pass rows (z, x) and
+// z-pass rows (y, x) at q = t + NT*k, x fastest.
+CZW_DI void ypass_bases(int t, int (&base)[NROW])
+{
+#pragma unroll
+    for (int k = 0; k < NROW; ++k) base[k] = ((t + NT * k) >> 5) * SP + ((t + NT * k) & 31);
+}
 
 template <int WT>
 CZW_DI void iwt_most_f32(float *A, int t)
 {
     iwt16_8<WT, RP, SP>(A, t);
+    {  // level 16 z-pass, then level 32 z-pass: rows (y, x)
+        float r[NROW][32];
 #pragma unroll
-    for (int k = 0; k < NROW; ++k) {  // level 16 z-pass, then level 32 z-pass
-        const int q = t + NT * k, y = q >> 5, x = q & 31;
-        float r[32];
-        lds_ld<32>(A, y * RP + x, SP, r);
-        if (y < 16 && x < 16) row<16, false, WT>(r);
-        row<32, false, WT>(r);
-        lds_st<32>(A, y * RP + x, SP, r);
+        for (int k = 0; k < NROW; ++k) lds_ld<32>(A, ((t + NT * k) >> 5) * RP + ((t + NT * k) & 31), SP, r[k]);
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) {
+            const int q = t + NT * k, y = q >> 5, x = q & 31;
+            if (y < 16 && x < 16) row<16, false, WT>(r[k]);
+            row<32, false, WT>(r[k]);
+            lds_st<32>(A, y * RP + x, SP, r[k]);
+        }
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NROW; ++k) {  // level 32 y-pass
-        const int q = t + NT * k, z = q >> 5, x = q & 31;
-        float r[32];
-        lds_ld<32>(A, z * SP + x, RP, r);
-        row<32, false, WT>(r);
-        lds_st<32>(A, z * SP + x, RP, r);
+    {  // level 32 y-pass
+        int base[NROW];
+        ypass_bases(t, base);
+        pass_rows_f32<false, WT>(A, base, RP);
     }
     __syncthreads();
 }
@@ -167,6 +165,12 @@ CZW_DI void load_xrow_f32(const float *__restrict__ src, const FieldGeom &g, int
             r[4 * i + 2] = v.z;
             r[4 * i + 3] = v.w;
         }
+    } else if constexpr (LAYOUT == LAYOUT_FIELD) {
+        size_t xs;
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(src + field_row(g, b, 0, 0, &xs));
+        const int vo = ((q >> 5) * g.nz + (q & 31)) * 4, so = (int)xs * 4;
+#pragma unroll
+        for (int x = 0; x < 32; ++x) r[x] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, x * so, 0));
     } else {
         size_t xs;
         const float *p = src + field_row(g, b, q >> 5, q & 31, &xs);
@@ -282,13 +286,10 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
             eps_b = br.eps;
             load_xrow_f32<RL>(br.src, br.g, br.lb, t, nx[0]);
         }
-#pragma unroll
-        for (int k = 0; k < NROW; ++k) {  // level 32, y-pass: row (z, x)
-            const int q = t + NT * k, z = q >> 5, x = q & 31;
-            float r[32];
-            lds_ld<32>(A, z * SP + x, RP, r);
-            row<32, true, WT>(r);
-            lds_st<32>(A, z * SP + x, RP, r);
+        {  // level 32, y-pass: rows (z, x)
+            int base[NROW];
+            ypass_bases(t, base);
+            pass_rows_f32<true, WT>(A, base, RP);
         }
         __syncthreads();
         // level 32, z-pass: row (y, x) = physical mask word x + 32*y, bit z
@@ -299,10 +300,11 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
         float r[NROW][32];
         uint32_t kb[NROW];
 #pragma unroll
+        for (int k = 0; k < NROW; ++k) lds_ld<32>(A, ((t + NT * k) >> 5) * RP + ((t + NT * k) & 31), SP, r[k]);
+#pragma unroll
         for (int k = 0; k < NROW; ++k) {
             const int q = t + NT * k, y = q >> 5, x = q & 31;
             const bool corner = (y < 16) && (x < 16);
-            lds_ld<32>(A, y * RP + x, SP, r[k]);
             row<32, true, WT>(r[k]);
             uint32_t bits = 0;
 #pragma unroll
@@ -518,15 +520,20 @@ __global__ __launch_bounds__(256) void k_pack_records(const uint8_t *__restrict_
 // flight and offsets[b+2G] is being fetched.
 constexpr int PF = 8;
 
+// Survivor words sv[k] = survivor k*NT + t were loaded for the indices below
+// `lim`; the record's extent is taken from the next record's offset when the
+// records are in order (a hint only: survivors past lim are read later), so
+// the prefetch does not read into the following records.
 struct RecPrefetch {
-    uint32_t nb, mw[NROW], sv[PF];
+    uint32_t nb, lim, mw[NROW], sv[PF];
 };
 
-CZW_DI void fetch_record(const uint8_t *__restrict__ records, uint64_t rec_bytes, uint64_t ob, int t,
-                         RecPrefetch &r)
+CZW_DI void fetch_record(const uint8_t *__restrict__ records, uint64_t rec_bytes, uint64_t ob, uint64_t ob_end,
+                         int t, RecPrefetch &r)
 {
     if ((ob & 3u) || ob + 4 + MASK_BYTES > rec_bytes) {
         r.nb = 0xffffffffu;  // rejected by the size check
+        r.lim = 0u;
 #pragma unroll
         for (int k = 0; k < NROW; ++k) r.mw[k] = 0u;
 #pragma unroll
@@ -534,14 +541,16 @@ CZW_DI void fetch_record(const uint8_t *__restrict__ records, uint64_t rec_bytes
         return;
     }
     const uint32_t *p = reinterpret_cast<const uint32_t *>(records + ob);
-    const uint64_t nwords = (rec_bytes - ob) / 4;  // words of the buffer from this record on
+    uint64_t nwords = (rec_bytes - ob) / 4;  // words of the buffer from this record on
+    if (ob_end > ob + 4 + MASK_BYTES && (ob_end - ob) / 4 < nwords) nwords = (ob_end - ob) / 4;
+    r.lim = nwords > 1 + 1024 ? (uint32_t)min(nwords - 1 - 1024, (uint64_t)(PF * NT)) : 0u;
     r.nb = p[0];
 #pragma unroll
     for (int k = 0; k < NROW; ++k) r.mw[k] = p[1 + t + NT * k];
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
-        const uint64_t i = 1 + 1024 + (uint64_t)k * NT + t;
-        r.sv[k] = i < nwords ? p[i] : 0u;
+        const uint32_t i = (uint32_t)(k * NT + t);
+        r.sv[k] = i < r.lim ? p[1 + 1024 + i] : 0u;
     }
 }
 
@@ -589,7 +598,7 @@ CZW_DI bool decode_block_fast_f32(float *A, uint32_t *M, uint32_t *cnt, uint32_t
 #pragma unroll
     for (int k = 0; k < PF; ++k) {
         const uint32_t i = (uint32_t)(k * NT + t);
-        if (i < S) stage[i] = cur.sv[k];
+        if (i < S) stage[i] = i < cur.lim ? cur.sv[k] : pay[MASK_BYTES / 4 + i];
     }
     for (uint32_t i = PF * NT + t; i < S; i += NT) stage[i] = pay[MASK_BYTES / 4 + i];
     if (t < 64) stage[STAGE32 + t] = 0u;  // the dummy words read for dropped coefficients
@@ -697,11 +706,13 @@ __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict
     const int t = threadIdx.x;
     const int G = gridDim.x;
     load_table<LAYOUT>(T, fields, nf);
+    // offset of a block's record and the next block's (an extent hint; 0 = none)
+    auto off_at = [&](int i) -> uint64_t { return i < nblocks ? offsets[i] : 0; };
     int b = blockIdx.x, b1 = b + G, b2 = b + 2 * G, bq = 0;
-    uint64_t ob = b < nblocks ? offsets[b] : 0;
-    uint64_t ob_next = b1 < nblocks ? offsets[b1] : 0;
+    uint64_t ob0 = off_at(b);                          // block b
+    uint64_t ob1 = off_at(b1), ob1_end = off_at(b1 + 1);  // block b1
     RecPrefetch pf;
-    if (b < nblocks) fetch_record(records, rec_bytes, ob, t, pf);
+    if (b < nblocks) fetch_record(records, rec_bytes, ob0, off_at(b + 1), t, pf);
     for (int it = 0; b < nblocks; ++it, b = b1, b1 = b2, b2 = bq) {
         if (queue) {
             if (t == 0) ticket[it & 1] = 3 * G + (int)atomicAdd(queue, 1u);
@@ -711,10 +722,13 @@ __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict
             bq = b2 + G;
         }
         const RecPrefetch cur = pf;
-        const uint64_t ob_cur = ob;
-        ob = ob_next;
-        if (b2 < nblocks) ob_next = offsets[b2];
-        if (b1 < nblocks) fetch_record(records, rec_bytes, ob, t, pf);
+        const uint64_t ob_cur = ob0;
+        if (b1 < nblocks) fetch_record(records, rec_bytes, ob1, ob1_end, t, pf);
+        ob0 = ob1;
+        if (b2 < nblocks) {
+            ob1 = offsets[b2];
+            ob1_end = off_at(b2 + 1);
+        }
 
         const uint32_t nb = cur.nb;
         if (nb < MASK_BYTES || ((nb - MASK_BYTES) & 3u) || nb > SLOT32 || ob_cur + 4 + nb > rec_bytes) {
@@ -731,16 +745,17 @@ __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict
                 continue;
             }
             iwt16_8<WT, RP, SP>(A, t);
+            // level 16 z-pass (corner columns), then level 32 z-pass; both
+            // threads' corner loads are issued before the first store
 #pragma unroll
-            for (int k = 0; k < NROW; ++k) {  // level 16 z-pass (corner columns), then level 32 z-pass
+            for (int k = 0; k < NROW; ++k) {
                 const int q = t + NT * k, y = q >> 5, x = q & 31;
-                if (y < 16 && x < 16) {
-                    float c[16];
-                    lds_ld<16>(A, y * RP + x, SP, c);
-                    row<16, false, WT>(c);
+                if (y < 16 && x < 16) lds_ld<16>(A, y * RP + x, SP, r[k]);
+            }
 #pragma unroll
-                    for (int z = 0; z < 16; ++z) r[k][z] = c[z];
-                }
+            for (int k = 0; k < NROW; ++k) {
+                const int q = t + NT * k, y = q >> 5, x = q & 31;
+                if (y < 16 && x < 16) row<16, false, WT>(r[k]);
                 row<32, false, WT>(r[k]);
                 lds_st<32>(A, y * RP + x, SP, r[k]);
             }
@@ -753,7 +768,10 @@ __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict
             for (int k = 0; k < NROW; ++k) M[t + NT * k] = cur.mw[k];
             uint32_t *stage = reinterpret_cast<uint32_t *>(A);
 #pragma unroll
-            for (int k = 0; k < PF; ++k) stage[k * NT + t] = cur.sv[k];
+            for (int k = 0; k < PF; ++k) {
+                const uint32_t i = (uint32_t)(k * NT + t);  // < S: dense blocks hold more than PF*NT survivors
+                stage[i] = i < cur.lim ? cur.sv[k] : pay[MASK_BYTES / 4 + i];
+            }
             for (uint32_t i = PF * NT + t; i < S; i += NT) stage[i] = pay[MASK_BYTES / 4 + i];
             __syncthreads();
             if (t < 16 && (M[(t & 3) + 32 * (t >> 2)] & 0xfu) != 0xfu) flag = 0u;
@@ -807,15 +825,10 @@ __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict
             __syncthreads();
             iwt_most_f32<WT>(A, t);
         }
-        if (S <= (uint32_t)STAGE32) {
-#pragma unroll
-            for (int k = 0; k < NROW; ++k) {  // level 32 y-pass
-                const int q = t + NT * k, z = q >> 5, x = q & 31;
-                float rr[32];
-                lds_ld<32>(A, z * SP + x, RP, rr);
-                row<32, false, WT>(rr);
-                lds_st<32>(A, z * SP + x, RP, rr);
-            }
+        if (S <= (uint32_t)STAGE32) {  // level 32 y-pass
+            int base[NROW];
+            ypass_bases(t, base);
+            pass_rows_f32<false, WT>(A, base, RP);
             __syncthreads();
         }
         if constexpr (RL == LAYOUT_BLOCKS) {
@@ -830,25 +843,37 @@ __global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict
             // level 32 x-pass back into A, then 16-byte stores along the
             // field's contiguous Z: four times fewer store instructions than
             // storing each x-row element on its own
+            {
+                int base[NROW];
 #pragma unroll
-            for (int k = 0; k < NROW; ++k) {
-                float rr[32];
-                const int base = (t & 31) * SP + ((t + NT * k) >> 5) * RP;
-                lds_ld<32>(A, base, 1, rr);
-                row<32, false, WT>(rr);
-                lds_st<32>(A, base, 1, rr);
+                for (int k = 0; k < NROW; ++k) base[k] = (t & 31) * SP + ((t + NT * k) >> 5) * RP;
+                pass_rows_f32<false, WT>(A, base, 1);
             }
             __syncthreads();
             const BlockRef<float> br = locate<LAYOUT, float>(T, nf, b, nullptr, dst, g, 0.0f);
             size_t xs;
             float *p0 = br.dst + field_row(br.g, br.lb, 0, 0, &xs);
             const int nz = br.g.nz;
+            if constexpr (RL == LAYOUT_FIELD) {
+                // x = i >> 8 is the same for the 64 lanes of a wave: a scalar offset
+                const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(p0);
+                const int so = (int)xs * 4;
 #pragma unroll 4
-            for (int i = t; i < 32 * 32 * 8; i += NT) {  // (x, y, z4) with z4 fastest
-                const int z4 = i & 7, y = (i >> 3) & 31, x = i >> 8;
-                const int a = (4 * z4) * SP + y * RP + x;
-                const float4 v = make_float4(A[a], A[a + SP], A[a + 2 * SP], A[a + 3 * SP]);
-                *reinterpret_cast<float4 *>(p0 + x * xs + (size_t)y * nz + 4 * z4) = v;
+                for (int i = t; i < 32 * 32 * 8; i += NT) {  // (x, y, z4) with z4 fastest
+                    const int z4 = i & 7, y = (i >> 3) & 31, x = __builtin_amdgcn_readfirstlane(i >> 8);
+                    const int a = (4 * z4) * SP + y * RP + x;
+                    const czw_u32x4 v = {__float_as_uint(A[a]), __float_as_uint(A[a + SP]), __float_as_uint(A[a + 2 * SP]),
+                                         __float_as_uint(A[a + 3 * SP])};
+                    __builtin_amdgcn_raw_buffer_store_b128(v, rs, (y * nz + 4 * z4) * 4, x * so, 0);
+                }
+            } else {
+#pragma unroll 4
+                for (int i = t; i < 32 * 32 * 8; i += NT) {  // (x, y, z4) with z4 fastest
+                    const int z4 = i & 7, y = (i >> 3) & 31, x = i >> 8;
+                    const int a = (4 * z4) * SP + y * RP + x;
+                    const float4 v = make_float4(A[a], A[a + SP], A[a + 2 * SP], A[a + 3 * SP]);
+                    *reinterpret_cast<float4 *>(p0 + x * xs + (size_t)y * nz + 4 * z4) = v;
+                }
             }
         }
     }
@@ -1036,11 +1061,35 @@ int czw_compress_field(const void *field, int nx, int ny, int nz, int prec, floa
     if (!field_ok(nx, ny, nz)) return CZW_EINVAL;
     const int nb = (nx / 32) * (ny / 32) * (nz / 32);
     const FieldGeom g{nx, ny, nz};
+    const bool buf = field_fits_buffer(ny, nz, prec);
     if (prec == 4)
-        return compress_f32<LAYOUT_FIELD>(static_cast<const float *>(field), nb, g, eps, wtype, zerobits, shuffle,
-                                          records, offsets, workspace, stream);
-    if (prec == 8) return compress_f64(LAYOUT_FIELD, field, nb, g, eps, wtype, zerobits, shuffle, records, offsets,
-                                       workspace, stream);
+        return buf ? compress_f32<LAYOUT_FIELD>(static_cast<const float *>(field), nb, g, eps, wtype, zerobits,
+                                                shuffle, records, offsets, workspace, stream)
+                   : compress_f32<LAYOUT_FIELD_WIDE>(static_cast<const float *>(field), nb, g, eps, wtype, zerobits,
+                                                     shuffle, records, offsets, workspace, stream);
+    if (prec == 8)
+        return compress_f64(buf ? LAYOUT_FIELD : LAYOUT_FIELD_WIDE, field, nb, g, eps, wtype, zerobits, shuffle,
+                            records, offsets, workspace, stream);
+    return CZW_EINVAL;
+}
+
+int czw_compress_field_range(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype,
+                             int zerobits, int shuffle, int block0, int nblocks, void *records, uint64_t *offsets,
+                             void *workspace, hipStream_t stream)
+{
+    if (!field_ok(nx, ny, nz)) return CZW_EINVAL;
+    const int total = (nx / 32) * (ny / 32) * (nz / 32);
+    if (block0 < 0 || nblocks <= 0 || block0 > total - nblocks) return CZW_EINVAL;
+    const FieldGeom g{nx, ny, nz, block0};
+    const bool buf = field_fits_buffer(ny, nz, prec);
+    if (prec == 4)
+        return buf ? compress_f32<LAYOUT_FIELD>(static_cast<const float *>(field), nblocks, g, eps, wtype, zerobits,
+                                                shuffle, records, offsets, workspace, stream)
+                   : compress_f32<LAYOUT_FIELD_WIDE>(static_cast<const float *>(field), nblocks, g, eps, wtype,
+                                                     zerobits, shuffle, records, offsets, workspace, stream);
+    if (prec == 8)
+        return compress_f64(buf ? LAYOUT_FIELD : LAYOUT_FIELD_WIDE, field, nblocks, g, eps, wtype, zerobits, shuffle,
+                            records, offsets, workspace, stream);
     return CZW_EINVAL;
 }
 
@@ -1050,12 +1099,15 @@ int czw_compress_field_slots(const void *field, int nx, int ny, int nz, int prec
     if (!field_ok(nx, ny, nz)) return CZW_EINVAL;
     const int nb = (nx / 32) * (ny / 32) * (nz / 32);
     const FieldGeom g{nx, ny, nz};
+    const bool buf = field_fits_buffer(ny, nz, prec);
     if (prec == 4)
-        return compress_slots_f32<LAYOUT_FIELD>(static_cast<const float *>(field), nb, g, eps, wtype, zerobits,
-                                                static_cast<uint8_t *>(slots), nbytes, stream);
+        return buf ? compress_slots_f32<LAYOUT_FIELD>(static_cast<const float *>(field), nb, g, eps, wtype, zerobits,
+                                                      static_cast<uint8_t *>(slots), nbytes, stream)
+                   : compress_slots_f32<LAYOUT_FIELD_WIDE>(static_cast<const float *>(field), nb, g, eps, wtype,
+                                                           zerobits, static_cast<uint8_t *>(slots), nbytes, stream);
     if (prec == 8)
-        return compress_slots_f64(LAYOUT_FIELD, field, nb, g, eps, wtype, zerobits, static_cast<uint8_t *>(slots),
-                                  nbytes, nullptr, 0, 0, nullptr, stream);
+        return compress_slots_f64(buf ? LAYOUT_FIELD : LAYOUT_FIELD_WIDE, field, nb, g, eps, wtype, zerobits,
+                                  static_cast<uint8_t *>(slots), nbytes, nullptr, 0, 0, nullptr, stream);
     return CZW_EINVAL;
 }
 
@@ -1090,12 +1142,15 @@ int czw_decompress_field(const void *records, size_t records_bytes, const uint64
     if (!field_ok(nx, ny, nz)) return CZW_EINVAL;
     const int nb = (nx / 32) * (ny / 32) * (nz / 32);
     const FieldGeom g{nx, ny, nz};
+    const bool buf = field_fits_buffer(ny, nz, prec);
     if (prec == 4)
-        return decompress_f32<LAYOUT_FIELD>(records, records_bytes, offsets, nb, g, wtype, shuffle,
-                                            static_cast<float *>(field), err, stream);
+        return buf ? decompress_f32<LAYOUT_FIELD>(records, records_bytes, offsets, nb, g, wtype, shuffle,
+                                                  static_cast<float *>(field), err, stream)
+                   : decompress_f32<LAYOUT_FIELD_WIDE>(records, records_bytes, offsets, nb, g, wtype, shuffle,
+                                                       static_cast<float *>(field), err, stream);
     if (prec == 8)
-        return decompress_f64(LAYOUT_FIELD, records, records_bytes, offsets, nb, g, wtype, shuffle, field, err,
-                              nullptr, 0, 0, nullptr, stream);
+        return decompress_f64(buf ? LAYOUT_FIELD : LAYOUT_FIELD_WIDE, records, records_bytes, offsets, nb, g, wtype,
+                              shuffle, field, err, nullptr, 0, 0, nullptr, stream);
     return CZW_EINVAL;
 }
 
@@ -1112,13 +1167,15 @@ int czw_decompress_blocks(const void *records, size_t records_bytes, const uint6
     return CZW_EINVAL;
 }
 
-int czw_batch_prepare(czw_field_t *fields, int nfields, int *nblocks)
+int czw_batch_prepare(czw_field_t *fields, int nfields, int prec, int *nblocks)
 {
-    if (!fields || !nblocks || nfields <= 0 || nfields > BATCH_MAX_FIELDS) return CZW_EINVAL;
+    if (!fields || !nblocks || nfields <= 0 || nfields > BATCH_MAX_FIELDS || (prec != 4 && prec != 8))
+        return CZW_EINVAL;
     int64_t total = 0;
     for (int f = 0; f < nfields; ++f) {
         czw_field_t &d = fields[f];
-        if (!field_ok(d.nx, d.ny, d.nz) || !(d.eps >= 0.0f)) return CZW_EINVAL;
+        if (!field_ok(d.nx, d.ny, d.nz) || !field_fits_buffer(d.ny, d.nz, prec) || !(d.eps >= 0.0f))
+            return CZW_EINVAL;
         d.block0 = (int32_t)total;
         d.nblocks = (d.nx / 32) * (d.ny / 32) * (d.nz / 32);
         total += d.nblocks;

@@ -41,6 +41,8 @@ def load_library():
         L.czh_gather.argtypes = [vp, i, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.POINTER(C.c_void_p)]
         L.czh_free.argtypes = [vp]
         L.czh_deflate.argtypes = [vp, C.c_uint64, vp, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.czh_compress_device_field.argtypes = [C.c_char_p, vp, i, i, i, i, C.c_float, i, i, i, i, i, i, i, vp,
+                                                C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -61,6 +63,32 @@ def write(path, records, offsets, bpd, prec, wtype, threshold, encoder="zlib", n
                                   ENCODERS[encoder], rec.ctypes.data, off.ctypes.data, nblocks, int(nthreads))
     if rc:
         raise CzFileError(f"czh_write failed ({rc})")
+
+
+def compress_device_field(path, field, eps, wtype=3, zerobits=0, shuffle=False, encoder="zlib", nthreads=0,
+                          slab_blocks=0, overlap=True):
+    """Device field (torch, HDF5 order [nx][ny][nz]) -> .cz file, the host zlib
+    stage of one slab of chunks overlapped with the device compressor on the
+    next.  path None: only the file size is computed.  Returns
+    (file_bytes, {wall, device, zlib, copy, assemble} seconds)."""
+    import torch
+
+    if field.dim() != 3 or not field.is_cuda or not field.is_contiguous():
+        raise CzFileError("field must be a contiguous 3D device tensor")
+    prec = {torch.float32: 4, torch.float64: 8}.get(field.dtype)
+    if prec is None:
+        raise CzFileError(f"unsupported dtype {field.dtype}")
+    nx, ny, nz = (int(v) for v in field.shape)
+    nbytes = C.c_uint64()
+    times = (C.c_double * 5)()
+    rc = load_library().czh_compress_device_field(
+        os.fsencode(path) if path else None, field.data_ptr(), nx, ny, nz, prec, float(eps), int(wtype),
+        int(zerobits), int(bool(shuffle)), ENCODERS[encoder], int(nthreads), int(slab_blocks), int(bool(overlap)),
+        C.c_void_p(torch.cuda.current_stream().cuda_stream), C.byref(nbytes), times)
+    if rc:
+        raise CzFileError(f"czh_compress_device_field failed ({rc})")
+    keys = ("wall", "device", "zlib", "copy", "assemble")
+    return int(nbytes.value), {k: float(v) for k, v in zip(keys, times)}
 
 
 def deflate(data):

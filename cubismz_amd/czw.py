@@ -65,6 +65,8 @@ def load_library():
     L.czw_decompress_field.argtypes = [vp, sz, vp, i, i, i, i, i, i, vp, vp, vp]
     L.czw_decompress_blocks.restype = i
     L.czw_decompress_blocks.argtypes = [vp, sz, vp, i, i, i, i, vp, vp, vp]
+    L.czw_compress_field_range.restype = i
+    L.czw_compress_field_range.argtypes = [vp, i, i, i, i, f, i, i, i, i, i, vp, vp, vp, vp]
     L.czw_transform_blocks.restype = i
     L.czw_transform_blocks.argtypes = [vp, i, i, i, i, vp]
     _lib = L
@@ -136,6 +138,23 @@ def compress_field(field, eps, wtype=3, zerobits=0, shuffle=False, records=None,
                               int(bool(shuffle)), C.c_void_p(records.data_ptr()), C.c_void_p(offsets.data_ptr()),
                               C.c_void_p(workspace.data_ptr()), _stream(torch))
     _check(rc, "czw_compress_field")
+    return records, offsets
+
+
+def compress_field_range(field, eps, wtype, zerobits, shuffle, block0, nblocks):
+    """compress_field for the field's blocks [block0, block0 + nblocks) only."""
+    torch = _torch()
+    if field.dim() != 3 or not field.is_cuda or not field.is_contiguous():
+        raise CZWError("field must be a contiguous 3D device tensor")
+    nx, ny, nz = (int(s) for s in field.shape)
+    prec = _prec(field.dtype, torch)
+    records, offsets, workspace = _buffers(torch, field.device, nblocks, prec, None, None, None)
+    rc = load_library().czw_compress_field_range(C.c_void_p(field.data_ptr()), nx, ny, nz, prec, float(eps),
+                                                 int(wtype), int(zerobits), int(bool(shuffle)), int(block0),
+                                                 int(nblocks), C.c_void_p(records.data_ptr()),
+                                                 C.c_void_p(offsets.data_ptr()), C.c_void_p(workspace.data_ptr()),
+                                                 _stream(torch))
+    _check(rc, "czw_compress_field_range")
     return records, offsets
 
 

@@ -10,13 +10,15 @@
  *    (Compressor/source/Reader_WaveletCompression.h:255-560);
  *  - czh_gather: the inflate + record lookup of load_block2 (:635-698), done
  *    once per chunk for the whole file instead of once per block.
- * All buffers here are HOST memory.  The device side is include/czw.h.
+ * All buffers here are HOST memory (czh_compress_device_field excepted: it
+ * takes a device field).  The device side is include/czw.h.
  */
 #ifndef CZ_HOST_H
 #define CZ_HOST_H
 
 #include <stddef.h>
 #include <stdint.h>
+#include <hip/hip_runtime_api.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -52,6 +54,20 @@ void czh_free(void *p);
 
 /* The writer's per-chunk deflate (zlib level Z_DEFAULT_COMPRESSION, Z_FINISH). */
 int czh_deflate(const uint8_t *in, uint64_t n, uint8_t *out, uint64_t cap, uint64_t *produced);
+
+/* Compress an HDF5-ordered DEVICE field [nx][ny][nz] of Real (prec bytes)
+ * straight to a single-writer .cz file (path may be NULL: only the size is
+ * reported).  The field is cut into slabs of whole chunks of about
+ * slab_blocks blocks (<= 0: 1024).  With overlap != 0 the device compresses
+ * slab k+1 (czw_compress_field_range) while nthreads host threads deflate
+ * slab k; with overlap == 0 the two alternate.  The bytes equal
+ * czh_write of the whole field's records.  times (may be NULL) receives 5
+ * values in seconds: wall, device compress, host zlib, device-to-host
+ * copies, file assembly and write.  Allocates its own device and pinned
+ * host buffers. */
+int czh_compress_device_field(const char *path, const void *field, int nx, int ny, int nz, int prec, float eps,
+                              int wtype, int zerobits, int shuffle, int encoder, int nthreads, int slab_blocks,
+                              int overlap, hipStream_t stream, uint64_t *file_bytes, double *times);
 
 #ifdef __cplusplus
 }

@@ -84,6 +84,14 @@ int czw_compress_field(const void *field, int nx, int ny, int nz, int prec, floa
                        int zerobits, int shuffle, void *records, uint64_t *offsets, void *workspace,
                        hipStream_t stream);
 
+/* czw_compress_field for the field's blocks [block0, block0 + nblocks) only
+ * (block numbering as in "Field layout"); records/offsets/workspace are sized
+ * for nblocks.  A writer cuts a field into such ranges to overlap the host
+ * zlib stage of one range with the device work of the next. */
+int czw_compress_field_range(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype,
+                             int zerobits, int shuffle, int block0, int nblocks, void *records, uint64_t *offsets,
+                             void *workspace, hipStream_t stream);
+
 /* The two stages of czw_compress_field, for callers that time the stages
  * apart or keep fixed-size per-block slots.  Stage 1 writes block b's mask and
  * survivors (zerobits applied, not yet shuffled) into slots +
@@ -155,8 +163,10 @@ typedef struct czw_batch {
     int queue;                 /* 1: dynamic block queue; 0: static stride */
 } czw_batch_t;
 
-/* Host: validate a table, fill block0/nblocks, return the total in *nblocks. */
-int czw_batch_prepare(czw_field_t *fields, int nfields, int *nblocks);
+/* Host: validate a table for Real of prec bytes, fill block0/nblocks, return
+ * the total in *nblocks.  Batch fields need 32*ny*nz*prec < 2^31 (the
+ * per-field entry points have no such limit). */
+int czw_batch_prepare(czw_field_t *fields, int nfields, int prec, int *nblocks);
 
 /* Scratch of a batch: compressor slots and sizes, then 256 B of queue counters. */
 size_t czw_batch_workspace_bytes(int nblocks, int prec);

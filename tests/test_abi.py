@@ -35,7 +35,7 @@ def test_batch_prepare_host_only():
 
     if not os.path.exists(czw.LIBPATH):
         pytest.skip("libczw.so not built")
-    t, total = B.prepare_table([(64, 32, 96), (32, 32, 32), (128, 64, 32)], [1e-3, 1e-4, 2e-2])
+    t, total = B.prepare_table([(64, 32, 96), (32, 32, 32), (128, 64, 32)], [1e-3, 1e-4, 2e-2], 4)
     assert list(t["block0"]) == [0, 6, 7]
     assert list(t["nblocks"]) == [6, 1, 8]
     assert total == 15
