@@ -99,8 +99,21 @@ def build_tools():
     return outs
 
 
+def build_checks():
+    """tests/native/adapter_check: the per-block C++ adapter (include/czw_compressor.hpp)
+    driven like the reference's writer and reader; run by the GPU tests."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    ndir = os.path.join(ROOT, "tests", "native")
+    out = os.path.join(ndir, "adapter_check")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(rocm, "include"),
+          os.path.join(ndir, "adapter_check.cpp"), "-o", out + ".tmp", "-L" + LIBDIR, "-lczw",
+          "-L" + os.path.join(rocm, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN/" + os.path.relpath(LIBDIR, ndir)])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all():
-    return [build_hip(), build_host(), *build_tools()]
+    return [build_hip(), build_host(), *build_tools(), build_checks()]
 
 
 if __name__ == "__main__":
