@@ -34,7 +34,7 @@ def build_hip():
     os.makedirs(LIBDIR, exist_ok=True)
     out = os.path.join(LIBDIR, "libczw.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "czw_kernels.hip"), os.path.join(CSRC, "czw_f64.hip"),
+    _run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "czw_kernels.hip"), os.path.join(CSRC, "czw_half.hip"),
           os.path.join(CSRC, "czw_transform.hip"),
           "-o", tmp])
     os.replace(tmp, out)

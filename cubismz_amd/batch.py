@@ -17,7 +17,7 @@ from . import czw
 
 FIELD_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("nx", "<i4"), ("ny", "<i4"), ("nz", "<i4"),
                         ("eps", "<f4"), ("block0", "<i4"), ("nblocks", "<i4")])
-MAX_FIELDS = 128
+MAX_FIELDS = 64
 
 
 class _Batch(C.Structure):

@@ -23,7 +23,7 @@ constexpr int NROW = 1024 / NT;    // level-32 rows (and stream segments) per th
 // per-element address arithmetic).  Fields whose 32 x-planes exceed that
 // range use LAYOUT_FIELD_WIDE (64-bit addresses).
 enum { LAYOUT_BLOCKS = 0, LAYOUT_FIELD = 1, LAYOUT_BATCH = 2, LAYOUT_FIELD_WIDE = 3 };
-constexpr int BATCH_MAX_FIELDS = 128;  // field table entries one launch keeps in LDS
+constexpr int BATCH_MAX_FIELDS = 64;  // field table entries one launch keeps in LDS
 
 template <int N, typename R>
 CZW_DI void lds_ld(const R *A, int base, int stride, R *r)
@@ -316,9 +316,18 @@ int scan_records(const uint32_t *nbytes, int n, uint64_t *offsets, hipStream_t s
 // binary64: the corner and level-32 code counts).
 inline size_t size_words(int prec) { return prec == 8 ? 9 : 2; }
 
-// binary64 kernels (czw_f64.hip); layout is LAYOUT_BLOCKS, LAYOUT_FIELD or
-// LAYOUT_BATCH (then src/dst come from the field table); groups = 0 launches
-// every resident workgroup; queue = nullptr keeps the static block order.
+// Half-block kernels (czw_half.hip) for Real of prec bytes; layout is
+// LAYOUT_BLOCKS, LAYOUT_FIELD(_WIDE) or LAYOUT_BATCH (then src/dst come from
+// the field table); groups = 0 launches every resident workgroup; queue =
+// nullptr keeps the static block order.  The *_f64 forms are prec 8.
+int compress_slots_half(int prec, int layout, const void *src, int nblocks, FieldGeom g, float eps, int wtype,
+                        int zerobits, uint8_t *slots, uint32_t *nbytes, const czw_field_t *fields, int nf, int groups,
+                        uint32_t *queue, hipStream_t s);
+int pack_records_half(int prec, const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle,
+                      uint8_t *records, uint64_t *offsets, hipStream_t s);
+int decompress_half(int prec, int layout, const void *records, size_t rec_bytes, const uint64_t *offsets,
+                    int nblocks, FieldGeom g, int wtype, int shuffle, void *dst, uint32_t *err,
+                    const czw_field_t *fields, int nf, int groups, uint32_t *queue, hipStream_t s);
 int compress_slots_f64(int layout, const void *src, int nblocks, FieldGeom g, float eps, int wtype, int zerobits,
                        uint8_t *slots, uint32_t *nbytes, const czw_field_t *fields, int nf, int groups,
                        uint32_t *queue, hipStream_t s);

@@ -1,9 +1,8 @@
-// czw_f64.hip -- binary64 block compressor and decompressor: the reference's
-// double build (Real = double; WaveletCompressor.cpp:153-259 with
-// sizeof(DataType) = 8).
-//
-// A 32^3 block of doubles is 256 KB and does not fit in LDS (160 KB), so a
-// workgroup keeps only one x-half of the block there at a time:
+// czw_half.hip -- block compressor and decompressor that keep one x-half of a
+// 32^3 block in LDS at a time (WaveletCompressor.cpp:153-259), for Real =
+// double (the reference's double build: a 256 KB block does not fit in the
+// 160 KB of LDS) and Real = float (a 64 KB half lets two workgroups share a
+// CU, so one's memory phases overlap the other's arithmetic):
 //
 //   compress:   level-32 x-pass on the thread's two x-rows in registers;
 //               x >= 16 half -> LDS: y-pass, z-pass, threshold, survivors
@@ -17,24 +16,26 @@
 // Because the two halves are handled one after the other, the survivors of
 // each half go to their own region of the block's slot with per-code counts
 // (see SLOT64 below); the packing kernel concatenates the regions in stream
-// order (codes 0..7, FullWaveletTransform.h:59-96).  The arithmetic is plain
-// binary64 without contraction (-ffp-contract=off), as in the reference.
+// order (codes 0..7, FullWaveletTransform.h:59-96).  The arithmetic is the
+// reference's: predictors in binary64, rounded to Real (czw_wavelet.hpp).
 //
 // LDS layout of a half: H[z][y][x & 15] at z*SPD + y*RPD + (x & 15) with
-// RPD = 17, SPD = 545 (doubles): x-rows for either thread order, y- and
-// z-rows and the corner passes are at most 2-way bank conflicted.
+// RPD = 17, SPD = 545 (elements): x-rows for either thread order, y- and
+// z-rows and the corner passes are at most 2-way bank conflicted for both
+// element sizes.
 #include "czw_common.hpp"
 
 namespace czw {
 
 constexpr int RPD = 17;
 constexpr int SPD = 32 * RPD + 1;
-constexpr int HALF_WORDS = 32 * SPD;  // doubles
-constexpr size_t SLOT64 = MASK_BYTES + 8 * (size_t)NB3;
-// Survivor regions of a slot, in doubles after the 4096-byte mask: the
+constexpr int HALF_WORDS = 32 * SPD;  // elements
+template <typename R>
+constexpr size_t slot_of() { return MASK_BYTES + sizeof(R) * (size_t)NB3; }
+// Survivor regions of a slot, in elements after the 4096-byte mask: the
 // [0,16)^3 corner (stream positions [0,4096), <= 4096 values), then the
 // x < 16 half's level-32 codes 1, 4, 5 (<= 3 x 4096), then the x >= 16
-// half's codes 2, 3, 6, 7 (<= 4 x 4096): 8 x 4096 doubles in all.
+// half's codes 2, 3, 6, 7 (<= 4 x 4096): 8 x 4096 elements in all.
 constexpr int REG_CORNER = 0;
 constexpr int REG_H0 = 4096;
 constexpr int REG_H1 = 4 * 4096;
@@ -50,66 +51,101 @@ CZW_DI int xrow_half_base(int q)
     else return (q & 31) * SPD + (q >> 5) * RPD;
 }
 
-template <int RL>
-CZW_DI void load_xrow_f64(const double *__restrict__ src, const FieldGeom &g, int b, int q, double *r)
+template <typename R>
+CZW_DI R from_bits(czw_u32x2 v);
+template <>
+CZW_DI double from_bits<double>(czw_u32x2 v)
+{
+    return __longlong_as_double((long long)(((unsigned long long)v.y << 32) | v.x));
+}
+
+// x in [x0, x0 + n) of x-row q into r[x0 .. x0 + n).
+template <typename R, int RL, int N = 32>
+CZW_DI void load_xrow_h(const R *__restrict__ src, const FieldGeom &g, int b, int q, R *r, int x0 = 0)
 {
     if constexpr (RL == LAYOUT_BLOCKS) {
-        const double2 *p = reinterpret_cast<const double2 *>(src + (size_t)b * NB3 + (size_t)q * 32);
+        const R *p = src + (size_t)b * NB3 + (size_t)q * 32 + x0;
+        if constexpr (sizeof(R) == 8) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const double2 v = p[i];
-            r[2 * i] = v.x;
-            r[2 * i + 1] = v.y;
+            for (int i = 0; i < N / 2; ++i) {
+                const double2 v = reinterpret_cast<const double2 *>(p)[i];
+                r[x0 + 2 * i] = v.x;
+                r[x0 + 2 * i + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < N / 4; ++i) {
+                const float4 v = reinterpret_cast<const float4 *>(p)[i];
+                r[x0 + 4 * i] = v.x;
+                r[x0 + 4 * i + 1] = v.y;
+                r[x0 + 4 * i + 2] = v.z;
+                r[x0 + 4 * i + 3] = v.w;
+            }
         }
     } else if constexpr (RL == LAYOUT_FIELD) {
         size_t xs;
         const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(src + field_row(g, b, 0, 0, &xs));
-        const int vo = ((q >> 5) * g.nz + (q & 31)) * 8, so = (int)xs * 8;
+        const int vo = ((q >> 5) * g.nz + (q & 31)) * (int)sizeof(R), so = (int)xs * (int)sizeof(R);
 #pragma unroll
-        for (int x = 0; x < 32; ++x) {
-            const czw_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, vo, x * so, 0);
-            r[x] = __longlong_as_double((long long)(((unsigned long long)v.y << 32) | v.x));
+        for (int x = x0; x < x0 + N; ++x) {
+            if constexpr (sizeof(R) == 8)
+                r[x] = from_bits<double>(__builtin_amdgcn_raw_buffer_load_b64(rs, vo, x * so, 0));
+            else
+                r[x] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, x * so, 0));
         }
     } else {
         size_t xs;
-        const double *p = src + field_row(g, b, q >> 5, q & 31, &xs);
+        const R *p = src + field_row(g, b, q >> 5, q & 31, &xs);
 #pragma unroll
-        for (int x = 0; x < 32; ++x) r[x] = p[x * xs];
+        for (int x = x0; x < x0 + N; ++x) r[x] = p[x * xs];
     }
 }
 
-template <int RL>
-CZW_DI void store_xrow_f64(double *__restrict__ dst, const FieldGeom &g, int b, int q, const double *r)
+template <typename R, int RL>
+CZW_DI void store_xrow_h(R *__restrict__ dst, const FieldGeom &g, int b, int q, const R *r)
 {
     if constexpr (RL == LAYOUT_BLOCKS) {
-        double2 *p = reinterpret_cast<double2 *>(dst + (size_t)b * NB3 + (size_t)q * 32);
+        R *p = dst + (size_t)b * NB3 + (size_t)q * 32;
+        if constexpr (sizeof(R) == 8) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) p[i] = make_double2(r[2 * i], r[2 * i + 1]);
+            for (int i = 0; i < 16; ++i) reinterpret_cast<double2 *>(p)[i] = make_double2(r[2 * i], r[2 * i + 1]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                reinterpret_cast<float4 *>(p)[i] = make_float4(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
+        }
     } else if constexpr (RL == LAYOUT_FIELD) {
         size_t xs;
         const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(dst + field_row(g, b, 0, 0, &xs));
-        const int vo = ((q >> 5) * g.nz + (q & 31)) * 8, so = (int)xs * 8;
+        const int vo = ((q >> 5) * g.nz + (q & 31)) * (int)sizeof(R), so = (int)xs * (int)sizeof(R);
 #pragma unroll
         for (int x = 0; x < 32; ++x) {
-            const unsigned long long u = (unsigned long long)__double_as_longlong(r[x]);
-            const czw_u32x2 v = {(unsigned)u, (unsigned)(u >> 32)};
-            __builtin_amdgcn_raw_buffer_store_b64(v, rs, vo, x * so, 0);
+            if constexpr (sizeof(R) == 8) {
+                const unsigned long long u = (unsigned long long)__double_as_longlong(r[x]);
+                const czw_u32x2 v = {(unsigned)u, (unsigned)(u >> 32)};
+                __builtin_amdgcn_raw_buffer_store_b64(v, rs, vo, x * so, 0);
+            } else {
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(r[x]), rs, vo, x * so, 0);
+            }
         }
     } else {
         size_t xs;
-        double *p = dst + field_row(g, b, q >> 5, q & 31, &xs);
+        R *p = dst + field_row(g, b, q >> 5, q & 31, &xs);
 #pragma unroll
         for (int x = 0; x < 32; ++x) p[x * xs] = r[x];
     }
 }
 
 // float_zero_bits3 (WaveletCompressor.cpp:34-43) on the first 32-bit word of
-// a little-endian double: the low mantissa word.
+// a little-endian Real: the whole float, or a double's low mantissa word.
 CZW_DI double zero_low_bits(double v, uint32_t zkeep)
 {
     const unsigned long long u = (unsigned long long)__double_as_longlong(v);
     return __longlong_as_double((long long)(u & (0xffffffff00000000ull | (unsigned long long)zkeep)));
 }
+CZW_DI float zero_low_bits(float v, uint32_t zkeep) { return __uint_as_float(__float_as_uint(v) & zkeep); }
+CZW_DI float absval(float v) { return fabsf(v); }
+CZW_DI double absval(double v) { return fabs(v); }
 
 // Exclusive scan of cnt[0..1024) in place (two entries per thread); returns
 // the total.  Barriers before and after are the caller's.
@@ -128,15 +164,15 @@ CZW_DI uint32_t scan1024(uint32_t *cnt, uint32_t *wsum, int t)
 // LAYOUT_BATCH field tables).  Block b's survivors go to the regions of slot
 // b, its payload size to nbytes[b] and its 8 region counts to
 // counts[8b .. 8b+8).
-template <int WT, int LAYOUT>
-__global__ __launch_bounds__(NT) void k_compress_f64(const double *__restrict__ src, FieldGeom g, int nblocks,
+template <typename R, int WT, int LAYOUT>
+__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_compress_h(const R *__restrict__ src, FieldGeom g, int nblocks,
                                                      float eps, uint32_t zkeep, uint8_t *__restrict__ slots,
                                                      uint32_t *__restrict__ nbytes, uint32_t *__restrict__ counts,
                                                      const czw_field_t *__restrict__ fields, int nf,
                                                      uint32_t *__restrict__ queue)
 {
     constexpr int RL = row_layout<LAYOUT>();
-    __shared__ double H[HALF_WORDS];
+    __shared__ R H[HALF_WORDS];
     __shared__ uint32_t M[1024];
     __shared__ uint32_t cnt[1024];
     __shared__ uint32_t wsum[NT / 64];
@@ -149,15 +185,15 @@ __global__ __launch_bounds__(NT) void k_compress_f64(const double *__restrict__ 
     const int xh = t & 15, sl = t >> 4;
     for (int b = blockIdx.x, it = 0; b < nblocks; ++it) {
         if (queue && t == 0) ticket[it & 1] = G + (int)atomicAdd(queue, 1u);
-        const BlockRef<double> br = locate<LAYOUT, double>(T, nf, b, src, nullptr, g, eps);
-        const double e = (double)br.eps;  // the double build compares against (double)eps
-        uint8_t *slot = slots + (size_t)b * SLOT64;
-        double *reg = reinterpret_cast<double *>(slot + MASK_BYTES);
-        double xr[NROW][32];
+        const BlockRef<R> br = locate<LAYOUT, R>(T, nf, b, src, nullptr, g, eps);
+        const R e = (R)br.eps;  // fabs(c) > threshold in the coefficient type (FullWaveletTransform.h:84)
+        uint8_t *slot = slots + (size_t)b * slot_of<R>();
+        R *reg = reinterpret_cast<R *>(slot + MASK_BYTES);
+        R xr[NROW][32];
 #pragma unroll
         for (int k = 0; k < NROW; ++k) {  // level 32, x-pass, in registers
-            load_xrow_f64<RL>(br.src, br.g, br.lb, t + NT * k, xr[k]);
-            row<32, true, WT, double>(xr[k]);
+            load_xrow_h<R, RL>(br.src, br.g, br.lb, t + NT * k, xr[k]);
+            row<32, true, WT, R>(xr[k]);
         }
 
         // ---- x >= 16: level-32 y- and z-passes; every coefficient is final
@@ -165,21 +201,21 @@ __global__ __launch_bounds__(NT) void k_compress_f64(const double *__restrict__ 
         for (int k = 0; k < NROW; ++k) lds_st<16>(H, xrow_half_base<RL>(t + NT * k), 1, &xr[k][16]);
         __syncthreads();
         {
-            double r[32];
+            R r[32];
             lds_ld<32>(H, sl * SPD + xh, RPD, r);  // y-pass: row (z, x)
-            row<32, true, WT, double>(r);
+            row<32, true, WT, R>(r);
             lds_st<32>(H, sl * SPD + xh, RPD, r);
         }
         __syncthreads();
         uint32_t S1;
         {
             const int y = sl, x = 16 + xh;
-            double r[32];
+            R r[32];
             lds_ld<32>(H, y * RPD + xh, SPD, r);  // z-pass: row (y, x)
-            row<32, true, WT, double>(r);
+            row<32, true, WT, R>(r);
             uint32_t bits = 0;
 #pragma unroll
-            for (int z = 0; z < 32; ++z) bits |= (uint32_t)(fabs(r[z]) > e) << z;
+            for (int z = 0; z < 32; ++z) bits |= (uint32_t)(absval(r[z]) > e) << z;
             M[x + 32 * y] = bits;
             // region order: codes 2, 3 (y < 16), 6, 7 (y >= 16); code bit 0 = z >= 16
             const int ri = 2 * (y >> 4), base = (y & 15) * 16 + xh;
@@ -209,9 +245,9 @@ __global__ __launch_bounds__(NT) void k_compress_f64(const double *__restrict__ 
         for (int k = 0; k < NROW; ++k) lds_st<16>(H, xrow_half_base<RL>(t + NT * k), 1, &xr[k][0]);
         __syncthreads();
         {
-            double r[32];
+            R r[32];
             lds_ld<32>(H, sl * SPD + xh, RPD, r);
-            row<32, true, WT, double>(r);
+            row<32, true, WT, R>(r);
             lds_st<32>(H, sl * SPD + xh, RPD, r);
         }
         __syncthreads();
@@ -219,17 +255,17 @@ __global__ __launch_bounds__(NT) void k_compress_f64(const double *__restrict__ 
         {
             const int y = sl, x = xh;
             const bool corner = y < 16;
-            double r[32];
+            R r[32];
             lds_ld<32>(H, y * RPD + xh, SPD, r);
-            row<32, true, WT, double>(r);
+            row<32, true, WT, R>(r);
             uint32_t bits = 0;
 #pragma unroll
-            for (int z = 0; z < 32; ++z) bits |= (uint32_t)(fabs(r[z]) > e) << z;
+            for (int z = 0; z < 32; ++z) bits |= (uint32_t)(absval(r[z]) > e) << z;
             if (corner) {
-                double c[16];
+                R c[16];
 #pragma unroll
                 for (int z = 0; z < 16; ++z) c[z] = r[z];
-                row<16, true, WT, double>(c);
+                row<16, true, WT, R>(c);
                 lds_st<16>(H, y * RPD + xh, SPD, c);
                 bits &= 0xffff0000u;  // z < 16 is decided with the corner below
             }
@@ -265,15 +301,15 @@ __global__ __launch_bounds__(NT) void k_compress_f64(const double *__restrict__ 
 
         // the [0,16)^3 corner: stream positions [0,4096) in pieces of 8 (base 4^3 always kept)
         uint32_t keep = 0;
-        double v[8];
+        R v[8];
         corner_piece<RPD, SPD>(t, [&](int q, int h, int addr, int stride, int word, int shift) {
             uint32_t bits = 0;
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 if (i < h) {
-                    const double c = H[addr + i * stride];
+                    const R c = H[addr + i * stride];
                     v[q * h + i] = c;
-                    bits |= (uint32_t)((t < 8) || (fabs(c) > e)) << i;
+                    bits |= (uint32_t)((t < 8) || (absval(c) > e)) << i;
                 }
             }
             keep |= bits << (q * h);
@@ -289,7 +325,7 @@ __global__ __launch_bounds__(NT) void k_compress_f64(const double *__restrict__ 
         for (int k = 0; k < NROW; ++k) reinterpret_cast<uint32_t *>(slot)[t + NT * k] = M[t + NT * k];
         if (t == 0) {
             counts[(size_t)NCNT * b] = Sc;
-            nbytes[b] = MASK_BYTES + 8u * (Sc + S0 + S1);
+            nbytes[b] = MASK_BYTES + (uint32_t)sizeof(R) * (Sc + S0 + S1);
         }
         const int bnext = queue ? ticket[it & 1] : b + G;
         __syncthreads();  // M, cnt, wsum and ticket are read before the next block reuses them
@@ -299,17 +335,19 @@ __global__ __launch_bounds__(NT) void k_compress_f64(const double *__restrict__ 
 
 // Frame each block's payload into the record stream: size word, mask, then
 // the survivors in stream order (corner, codes 1..7 from their regions), with
-// the byte-plane shuffle for sizeof(Real) = 8 if asked (WaveletCompressor.cpp:56-73).
+// the byte-plane shuffle for sizeof(Real) if asked (WaveletCompressor.cpp:56-73).
 // Records start at 4-byte offsets, so doubles are written as two words.
-__global__ __launch_bounds__(256) void k_pack_records_f64(const uint8_t *__restrict__ slots,
-                                                          const uint32_t *__restrict__ nbytes,
-                                                          const uint32_t *__restrict__ counts, int shuffle,
-                                                          const uint64_t *__restrict__ offsets,
-                                                          uint8_t *__restrict__ records)
+template <typename R>
+__global__ __launch_bounds__(256) void k_pack_records_h(const uint8_t *__restrict__ slots,
+                                                        const uint32_t *__restrict__ nbytes,
+                                                        const uint32_t *__restrict__ counts, int shuffle,
+                                                        const uint64_t *__restrict__ offsets,
+                                                        uint8_t *__restrict__ records)
 {
+    constexpr int W = (int)sizeof(R) / 4;  // words per survivor
     const int b = blockIdx.x, t = threadIdx.x;
     const uint32_t n = nbytes[b];
-    const uint32_t S = (n - MASK_BYTES) / 8;
+    const uint32_t S = (n - MASK_BYTES) / (uint32_t)sizeof(R);
     const uint32_t *cb = counts + (size_t)NCNT * b;
     uint32_t c[NCNT];
 #pragma unroll
@@ -322,8 +360,8 @@ __global__ __launch_bounds__(256) void k_pack_records_f64(const uint8_t *__restr
     st[0] = 0;
 #pragma unroll
     for (int i = 0; i < NCNT; ++i) st[i + 1] = st[i] + c[i];
-    const uint8_t *slot = slots + (size_t)b * SLOT64;
-    const unsigned long long *reg = reinterpret_cast<const unsigned long long *>(slot + MASK_BYTES);
+    const uint8_t *slot = slots + (size_t)b * slot_of<R>();
+    const uint32_t *reg = reinterpret_cast<const uint32_t *>(slot + MASK_BYTES);
     uint32_t *dst = reinterpret_cast<uint32_t *>(records + offsets[b]);
     uint8_t *dst8 = records + offsets[b] + 4 + MASK_BYTES;
     if (t == 0) dst[0] = n;
@@ -332,13 +370,16 @@ __global__ __launch_bounds__(256) void k_pack_records_f64(const uint8_t *__restr
         int seg = 0;
 #pragma unroll
         for (int j = 1; j < NCNT; ++j) seg += (i >= st[j]) ? 1 : 0;
-        const unsigned long long v = reg[from[seg] + (i - st[seg])];
+        const size_t e = from[seg] + (i - st[seg]);
+        uint32_t w[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) w[k] = reg[e * W + k];
         if (shuffle) {
 #pragma unroll
-            for (int p = 0; p < 8; ++p) dst8[(size_t)p * S + i] = (uint8_t)(v >> (8 * p));
+            for (int p = 0; p < 4 * W; ++p) dst8[(size_t)p * S + i] = (uint8_t)(w[p >> 2] >> (8 * (p & 3)));
         } else {
-            dst[1 + MASK_BYTES / 4 + 2 * i] = (uint32_t)v;
-            dst[1 + MASK_BYTES / 4 + 2 * i + 1] = (uint32_t)(v >> 32);
+#pragma unroll
+            for (int k = 0; k < W; ++k) dst[1 + MASK_BYTES / 4 + (size_t)W * i + k] = w[k];
         }
     }
 }
@@ -348,37 +389,49 @@ __global__ __launch_bounds__(256) void k_pack_records_f64(const uint8_t *__restr
 // 100-150 and the inverse transform), one block per workgroup iteration.
 // Survivors are read straight from the record (they are at 4-byte offsets;
 // byte planes when shuffled).
-struct Payload64 {
-    const uint32_t *w;  // unshuffled: survivor i = words 2i, 2i+1
+template <typename R>
+struct Payload {
+    const uint32_t *w;  // unshuffled: survivor i = words W*i .. W*i + W - 1
     const uint8_t *b8;  // shuffled: byte p of survivor i at p*S + i
     uint32_t S;
     int shuffle;
-    CZW_DI double get(uint32_t i) const
+    CZW_DI R get(uint32_t i) const
     {
-        unsigned long long u;
-        if (shuffle) {
-            u = 0;
+        if constexpr (sizeof(R) == 8) {
+            unsigned long long u;
+            if (shuffle) {
+                u = 0;
 #pragma unroll
-            for (int p = 0; p < 8; ++p) u |= (unsigned long long)b8[(size_t)p * S + i] << (8 * p);
+                for (int p = 0; p < 8; ++p) u |= (unsigned long long)b8[(size_t)p * S + i] << (8 * p);
+            } else {
+                u = (unsigned long long)w[2 * i] | ((unsigned long long)w[2 * i + 1] << 32);
+            }
+            return __longlong_as_double((long long)u);
         } else {
-            u = (unsigned long long)w[2 * i] | ((unsigned long long)w[2 * i + 1] << 32);
+            uint32_t u;
+            if (shuffle) {
+                u = (uint32_t)b8[i] | ((uint32_t)b8[S + i] << 8) | ((uint32_t)b8[2 * (size_t)S + i] << 16) |
+                    ((uint32_t)b8[3 * (size_t)S + i] << 24);
+            } else {
+                u = w[i];
+            }
+            return __uint_as_float(u);
         }
-        return __longlong_as_double((long long)u);
     }
 };
 
-template <int WT, int LAYOUT>
-__global__ __launch_bounds__(NT) void k_decompress_f64(const uint8_t *__restrict__ records, uint64_t rec_bytes,
+template <typename R, int WT, int LAYOUT>
+__global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(const uint8_t *__restrict__ records, uint64_t rec_bytes,
                                                        const uint64_t *__restrict__ offsets, int nblocks,
-                                                       int shuffle, double *__restrict__ dst, FieldGeom g,
+                                                       int shuffle, R *__restrict__ dst, FieldGeom g,
                                                        uint32_t *__restrict__ err,
                                                        const czw_field_t *__restrict__ fields, int nf,
                                                        uint32_t *__restrict__ queue)
 {
     constexpr int RL = row_layout<LAYOUT>();
-    __shared__ double H[HALF_WORDS];
+    __shared__ R H[HALF_WORDS];
     __shared__ uint32_t M[1024];
-    __shared__ uint32_t cnt[2560];  // corner pieces [0,512), level-32 runs 512 + (code-1)*256 + run
+    __shared__ uint16_t cnt[2560];  // corner pieces [0,512), level-32 runs 512 + (code-1)*256 + run
     __shared__ uint32_t wsum[NT / 64];
     __shared__ uint32_t flag;
     __shared__ int ticket[2];
@@ -403,12 +456,12 @@ __global__ __launch_bounds__(NT) void k_decompress_f64(const uint8_t *__restrict
         }
         const uint32_t *p = reinterpret_cast<const uint32_t *>(records + ob);
         const uint32_t nb = p[0];
-        if (nb < MASK_BYTES || ((nb - MASK_BYTES) & 7u) || nb > SLOT64 || ob + 4 + nb > rec_bytes) {
+        if (nb < MASK_BYTES || ((nb - MASK_BYTES) % sizeof(R)) || nb > slot_of<R>() || ob + 4 + nb > rec_bytes) {
             if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_NBYTES);
             continue;
         }
-        const uint32_t S = (nb - MASK_BYTES) >> 3;
-        const Payload64 pay{p + 1 + MASK_BYTES / 4, records + ob + 4 + MASK_BYTES, S, shuffle};
+        const uint32_t S = (nb - MASK_BYTES) / (uint32_t)sizeof(R);
+        const Payload<R> pay{p + 1 + MASK_BYTES / 4, records + ob + 4 + MASK_BYTES, S, shuffle};
         __syncthreads();  // the previous block's last reads of H, M, cnt and flag are done
         if (t == 0) flag = 1u;
 #pragma unroll
@@ -463,7 +516,7 @@ __global__ __launch_bounds__(NT) void k_decompress_f64(const uint8_t *__restrict
                 for (int i = 0; i < 8; ++i) {
                     if (i < h) {
                         const bool bit = (keep >> (q * h + i)) & 1u;
-                        H[addr + i * stride] = bit ? pay.get(idx) : 0.0;
+                        H[addr + i * stride] = bit ? pay.get(idx) : (R)0;
                         idx += bit;
                     }
                 }
@@ -476,18 +529,18 @@ __global__ __launch_bounds__(NT) void k_decompress_f64(const uint8_t *__restrict
         {
             const int y = sl, x = xh, base = (y & 15) * 16 + xh;
             const uint32_t mw = M[x + 32 * y];
-            double r[32];
+            R r[32];
             if (y < 16) {
-                double c[16];
+                R c[16];
                 lds_ld<16>(H, y * RPD + xh, SPD, c);
-                row<16, false, WT, double>(c);
+                row<16, false, WT, R>(c);
 #pragma unroll
                 for (int z = 0; z < 16; ++z) r[z] = c[z];
                 uint32_t i1 = cnt[512 + base];  // code 1
 #pragma unroll
                 for (int z = 16; z < 32; ++z) {
                     const bool bit = (mw >> z) & 1u;
-                    r[z] = bit ? pay.get(i1) : 0.0;
+                    r[z] = bit ? pay.get(i1) : (R)0;
                     i1 += bit;
                 }
             } else {
@@ -496,22 +549,22 @@ __global__ __launch_bounds__(NT) void k_decompress_f64(const uint8_t *__restrict
                 for (int z = 0; z < 32; ++z) {
                     uint32_t &i = z < 16 ? i0 : i1;
                     const bool bit = (mw >> z) & 1u;
-                    r[z] = bit ? pay.get(i) : 0.0;
+                    r[z] = bit ? pay.get(i) : (R)0;
                     i += bit;
                 }
             }
-            row<32, false, WT, double>(r);
+            row<32, false, WT, R>(r);
             lds_st<32>(H, y * RPD + xh, SPD, r);
         }
         __syncthreads();
         {
-            double r[32];
+            R r[32];
             lds_ld<32>(H, sl * SPD + xh, RPD, r);  // y-pass: row (z, x)
-            row<32, false, WT, double>(r);
+            row<32, false, WT, R>(r);
             lds_st<32>(H, sl * SPD + xh, RPD, r);
         }
         __syncthreads();
-        double lo[NROW][16];
+        R lo[NROW][16];
 #pragma unroll
         for (int k = 0; k < NROW; ++k) lds_ld<16>(H, xrow_half_base<RL>(t + NT * k), 1, lo[k]);
         __syncthreads();
@@ -522,36 +575,36 @@ __global__ __launch_bounds__(NT) void k_decompress_f64(const uint8_t *__restrict
             const int c0 = 2 + 4 * (y >> 4);
             const uint32_t mw = M[x + 32 * y];
             uint32_t i0 = cnt[512 + (c0 - 1) * 256 + base], i1 = cnt[512 + c0 * 256 + base];
-            double r[32];
+            R r[32];
 #pragma unroll
             for (int z = 0; z < 32; ++z) {
                 uint32_t &i = z < 16 ? i0 : i1;
                 const bool bit = (mw >> z) & 1u;
-                r[z] = bit ? pay.get(i) : 0.0;
+                r[z] = bit ? pay.get(i) : (R)0;
                 i += bit;
             }
-            row<32, false, WT, double>(r);
+            row<32, false, WT, R>(r);
             lds_st<32>(H, y * RPD + xh, SPD, r);
         }
         __syncthreads();
         {
-            double r[32];
+            R r[32];
             lds_ld<32>(H, sl * SPD + xh, RPD, r);
-            row<32, false, WT, double>(r);
+            row<32, false, WT, R>(r);
             lds_st<32>(H, sl * SPD + xh, RPD, r);
         }
         __syncthreads();
         // level-32 x-pass on whole rows in registers, then the store
-        const BlockRef<double> br = locate<LAYOUT, double>(T, nf, b, nullptr, dst, g, 0.0f);
+        const BlockRef<R> br = locate<LAYOUT, R>(T, nf, b, nullptr, dst, g, 0.0f);
 #pragma unroll
         for (int k = 0; k < NROW; ++k) {
             const int q = t + NT * k;
-            double r[32];
+            R r[32];
 #pragma unroll
             for (int i = 0; i < 16; ++i) r[i] = lo[k][i];
             lds_ld<16>(H, xrow_half_base<RL>(q), 1, &r[16]);
-            row<32, false, WT, double>(r);
-            store_xrow_f64<RL>(br.dst, br.g, br.lb, q, r);
+            row<32, false, WT, R>(r);
+            store_xrow_h<R, RL>(br.dst, br.g, br.lb, q, r);
         }
     }
 }
@@ -559,94 +612,147 @@ __global__ __launch_bounds__(NT) void k_decompress_f64(const uint8_t *__restrict
 // ---------------------------------------------------------------------------
 // host side
 
-template <int LAYOUT>
-static int launch_compress_f64(int wt, const double *src, FieldGeom g, int nblocks, float eps, uint32_t zkeep,
-                               uint8_t *slots, uint32_t *nb, uint32_t *cnt, const czw_field_t *fields, int nf,
+template <typename R, int LAYOUT>
+static int launch_compress_h(int wt, const R *src, FieldGeom g, int nblocks, float eps, uint32_t zkeep,
+                             uint8_t *slots, uint32_t *nb, uint32_t *cnt, const czw_field_t *fields, int nf,
+                             int groups, uint32_t *queue, hipStream_t s)
+{
+    const void *fn = wt == 1 ? (const void *)k_compress_h<R, 1, LAYOUT>
+                   : wt == 2 ? (const void *)k_compress_h<R, 2, LAYOUT> : (const void *)k_compress_h<R, 3, LAYOUT>;
+    const dim3 grid(std::min(nblocks, groups ? groups : resident_groups(fn, NT))), blk(NT);
+    if (wt == 1) hipLaunchKernelGGL((k_compress_h<R, 1, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, cnt, fields, nf, queue);
+    if (wt == 2) hipLaunchKernelGGL((k_compress_h<R, 2, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, cnt, fields, nf, queue);
+    if (wt == 3) hipLaunchKernelGGL((k_compress_h<R, 3, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, cnt, fields, nf, queue);
+    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
+}
+
+template <typename R>
+static int compress_slots_t(int layout, const void *src, int nblocks, FieldGeom g, float eps, int wt,
+                            uint32_t zkeep, uint8_t *slots, uint32_t *nbytes, const czw_field_t *fields, int nf,
+                            int groups, uint32_t *queue, hipStream_t s)
+{
+    uint32_t *cnt = nbytes + nblocks;
+    const R *d = static_cast<const R *>(src);
+    switch (layout) {
+    case LAYOUT_BLOCKS:
+        return launch_compress_h<R, LAYOUT_BLOCKS>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, nullptr, 0,
+                                                   groups, queue, s);
+    case LAYOUT_FIELD:
+        return launch_compress_h<R, LAYOUT_FIELD>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, nullptr, 0,
+                                                  groups, queue, s);
+    case LAYOUT_FIELD_WIDE:
+        return launch_compress_h<R, LAYOUT_FIELD_WIDE>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, nullptr,
+                                                       0, groups, queue, s);
+    default:
+        return launch_compress_h<R, LAYOUT_BATCH>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, fields, nf,
+                                                  groups, queue, s);
+    }
+}
+
+int compress_slots_half(int prec, int layout, const void *src, int nblocks, FieldGeom g, float eps, int wtype,
+                        int zerobits, uint8_t *slots, uint32_t *nbytes, const czw_field_t *fields, int nf, int groups,
+                        uint32_t *queue, hipStream_t s)
+{
+    const int wt = wt_index(wtype);
+    if (wt < 0 || !zb_ok(zerobits) || nblocks <= 0 || !slots || !nbytes) return CZW_EINVAL;
+    if (layout != LAYOUT_BATCH && !src) return CZW_EINVAL;
+    const uint32_t zkeep = zerobits ? ~((1u << zerobits) - 1u) : 0xffffffffu;
+    if (prec == 8)
+        return compress_slots_t<double>(layout, src, nblocks, g, eps, wt, zkeep, slots, nbytes, fields, nf, groups,
+                                        queue, s);
+    if (prec == 4)
+        return compress_slots_t<float>(layout, src, nblocks, g, eps, wt, zkeep, slots, nbytes, fields, nf, groups,
+                                       queue, s);
+    return CZW_EINVAL;
+}
+
+int pack_records_half(int prec, const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle,
+                      uint8_t *records, uint64_t *offsets, hipStream_t s)
+{
+    if (nblocks <= 0 || !slots || !nbytes || !records || !offsets || (prec != 4 && prec != 8)) return CZW_EINVAL;
+    if (scan_records(nbytes, nblocks, offsets, s) != CZW_OK) return CZW_EHIP;
+    if (prec == 8)
+        hipLaunchKernelGGL(k_pack_records_h<double>, dim3(nblocks), dim3(256), 0, s, slots, nbytes, nbytes + nblocks,
+                           shuffle, offsets, records);
+    else
+        hipLaunchKernelGGL(k_pack_records_h<float>, dim3(nblocks), dim3(256), 0, s, slots, nbytes, nbytes + nblocks,
+                           shuffle, offsets, records);
+    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
+}
+
+template <typename R, int LAYOUT>
+static int launch_decompress_h(int wt, const uint8_t *rec, uint64_t rb, const uint64_t *offsets, int nblocks,
+                               int shuffle, R *dst, FieldGeom g, uint32_t *err, const czw_field_t *fields, int nf,
                                int groups, uint32_t *queue, hipStream_t s)
 {
-    const void *fn = wt == 1 ? (const void *)k_compress_f64<1, LAYOUT>
-                   : wt == 2 ? (const void *)k_compress_f64<2, LAYOUT> : (const void *)k_compress_f64<3, LAYOUT>;
+    const void *fn = wt == 1 ? (const void *)k_decompress_h<R, 1, LAYOUT>
+                   : wt == 2 ? (const void *)k_decompress_h<R, 2, LAYOUT> : (const void *)k_decompress_h<R, 3, LAYOUT>;
     const dim3 grid(std::min(nblocks, groups ? groups : resident_groups(fn, NT))), blk(NT);
-    if (wt == 1) hipLaunchKernelGGL((k_compress_f64<1, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, cnt, fields, nf, queue);
-    if (wt == 2) hipLaunchKernelGGL((k_compress_f64<2, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, cnt, fields, nf, queue);
-    if (wt == 3) hipLaunchKernelGGL((k_compress_f64<3, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, cnt, fields, nf, queue);
+    if (wt == 1) hipLaunchKernelGGL((k_decompress_h<R, 1, LAYOUT>), grid, blk, 0, s, rec, rb, offsets, nblocks, shuffle, dst, g, err, fields, nf, queue);
+    if (wt == 2) hipLaunchKernelGGL((k_decompress_h<R, 2, LAYOUT>), grid, blk, 0, s, rec, rb, offsets, nblocks, shuffle, dst, g, err, fields, nf, queue);
+    if (wt == 3) hipLaunchKernelGGL((k_decompress_h<R, 3, LAYOUT>), grid, blk, 0, s, rec, rb, offsets, nblocks, shuffle, dst, g, err, fields, nf, queue);
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
+}
+
+template <typename R>
+static int decompress_t(int layout, const uint8_t *rec, uint64_t rb, const uint64_t *offsets, int nblocks,
+                        FieldGeom g, int wt, int shuffle, void *dst, uint32_t *err, const czw_field_t *fields, int nf,
+                        int groups, uint32_t *queue, hipStream_t s)
+{
+    R *d = static_cast<R *>(dst);
+    switch (layout) {
+    case LAYOUT_BLOCKS:
+        return launch_decompress_h<R, LAYOUT_BLOCKS>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, nullptr, 0,
+                                                     groups, queue, s);
+    case LAYOUT_FIELD:
+        return launch_decompress_h<R, LAYOUT_FIELD>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, nullptr, 0,
+                                                    groups, queue, s);
+    case LAYOUT_FIELD_WIDE:
+        return launch_decompress_h<R, LAYOUT_FIELD_WIDE>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, nullptr,
+                                                         0, groups, queue, s);
+    default:
+        return launch_decompress_h<R, LAYOUT_BATCH>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, fields, nf,
+                                                    groups, queue, s);
+    }
+}
+
+int decompress_half(int prec, int layout, const void *records, size_t rec_bytes, const uint64_t *offsets,
+                    int nblocks, FieldGeom g, int wtype, int shuffle, void *dst, uint32_t *err,
+                    const czw_field_t *fields, int nf, int groups, uint32_t *queue, hipStream_t s)
+{
+    const int wt = wt_index(wtype);
+    if (wt < 0 || nblocks <= 0 || !records || !offsets || !err) return CZW_EINVAL;
+    if (layout != LAYOUT_BATCH && !dst) return CZW_EINVAL;
+    const uint8_t *rec = static_cast<const uint8_t *>(records);
+    if (prec == 8)
+        return decompress_t<double>(layout, rec, (uint64_t)rec_bytes, offsets, nblocks, g, wt, shuffle, dst, err,
+                                    fields, nf, groups, queue, s);
+    if (prec == 4)
+        return decompress_t<float>(layout, rec, (uint64_t)rec_bytes, offsets, nblocks, g, wt, shuffle, dst, err,
+                                   fields, nf, groups, queue, s);
+    return CZW_EINVAL;
 }
 
 int compress_slots_f64(int layout, const void *src, int nblocks, FieldGeom g, float eps, int wtype, int zerobits,
                        uint8_t *slots, uint32_t *nbytes, const czw_field_t *fields, int nf, int groups,
                        uint32_t *queue, hipStream_t s)
 {
-    const int wt = wt_index(wtype);
-    if (wt < 0 || !zb_ok(zerobits) || nblocks <= 0 || !slots || !nbytes) return CZW_EINVAL;
-    if (layout != LAYOUT_BATCH && !src) return CZW_EINVAL;
-    const uint32_t zkeep = zerobits ? ~((1u << zerobits) - 1u) : 0xffffffffu;
-    uint32_t *cnt = nbytes + nblocks;
-    const double *d = static_cast<const double *>(src);
-    switch (layout) {
-    case LAYOUT_BLOCKS:
-        return launch_compress_f64<LAYOUT_BLOCKS>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, nullptr, 0,
-                                                  groups, queue, s);
-    case LAYOUT_FIELD:
-        return launch_compress_f64<LAYOUT_FIELD>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, nullptr, 0,
-                                                 groups, queue, s);
-    case LAYOUT_FIELD_WIDE:
-        return launch_compress_f64<LAYOUT_FIELD_WIDE>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, nullptr, 0,
-                                                      groups, queue, s);
-    default:
-        return launch_compress_f64<LAYOUT_BATCH>(wt, d, g, nblocks, eps, zkeep, slots, nbytes, cnt, fields, nf,
-                                                 groups, queue, s);
-    }
+    return compress_slots_half(8, layout, src, nblocks, g, eps, wtype, zerobits, slots, nbytes, fields, nf, groups,
+                               queue, s);
 }
 
 int pack_records_f64(const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle, uint8_t *records,
                      uint64_t *offsets, hipStream_t s)
 {
-    if (nblocks <= 0 || !slots || !nbytes || !records || !offsets) return CZW_EINVAL;
-    if (scan_records(nbytes, nblocks, offsets, s) != CZW_OK) return CZW_EHIP;
-    hipLaunchKernelGGL(k_pack_records_f64, dim3(nblocks), dim3(256), 0, s, slots, nbytes, nbytes + nblocks, shuffle,
-                       offsets, records);
-    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
-}
-
-template <int LAYOUT>
-static int launch_decompress_f64(int wt, const uint8_t *rec, uint64_t rb, const uint64_t *offsets, int nblocks,
-                                 int shuffle, double *dst, FieldGeom g, uint32_t *err, const czw_field_t *fields,
-                                 int nf, int groups, uint32_t *queue, hipStream_t s)
-{
-    const void *fn = wt == 1 ? (const void *)k_decompress_f64<1, LAYOUT>
-                   : wt == 2 ? (const void *)k_decompress_f64<2, LAYOUT> : (const void *)k_decompress_f64<3, LAYOUT>;
-    const dim3 grid(std::min(nblocks, groups ? groups : resident_groups(fn, NT))), blk(NT);
-    if (wt == 1) hipLaunchKernelGGL((k_decompress_f64<1, LAYOUT>), grid, blk, 0, s, rec, rb, offsets, nblocks, shuffle, dst, g, err, fields, nf, queue);
-    if (wt == 2) hipLaunchKernelGGL((k_decompress_f64<2, LAYOUT>), grid, blk, 0, s, rec, rb, offsets, nblocks, shuffle, dst, g, err, fields, nf, queue);
-    if (wt == 3) hipLaunchKernelGGL((k_decompress_f64<3, LAYOUT>), grid, blk, 0, s, rec, rb, offsets, nblocks, shuffle, dst, g, err, fields, nf, queue);
-    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
+    return pack_records_half(8, slots, nbytes, nblocks, shuffle, records, offsets, s);
 }
 
 int decompress_f64(int layout, const void *records, size_t rec_bytes, const uint64_t *offsets, int nblocks,
                    FieldGeom g, int wtype, int shuffle, void *dst, uint32_t *err, const czw_field_t *fields, int nf,
                    int groups, uint32_t *queue, hipStream_t s)
 {
-    const int wt = wt_index(wtype);
-    if (wt < 0 || nblocks <= 0 || !records || !offsets || !err) return CZW_EINVAL;
-    if (layout != LAYOUT_BATCH && !dst) return CZW_EINVAL;
-    const uint8_t *rec = static_cast<const uint8_t *>(records);
-    double *d = static_cast<double *>(dst);
-    const uint64_t rb = (uint64_t)rec_bytes;
-    switch (layout) {
-    case LAYOUT_BLOCKS:
-        return launch_decompress_f64<LAYOUT_BLOCKS>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, nullptr, 0,
-                                                    groups, queue, s);
-    case LAYOUT_FIELD:
-        return launch_decompress_f64<LAYOUT_FIELD>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, nullptr, 0,
-                                                   groups, queue, s);
-    case LAYOUT_FIELD_WIDE:
-        return launch_decompress_f64<LAYOUT_FIELD_WIDE>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, nullptr,
-                                                        0, groups, queue, s);
-    default:
-        return launch_decompress_f64<LAYOUT_BATCH>(wt, rec, rb, offsets, nblocks, shuffle, d, g, err, fields, nf,
-                                                   groups, queue, s);
-    }
+    return decompress_half(8, layout, records, rec_bytes, offsets, nblocks, g, wtype, shuffle, dst, err, fields, nf,
+                           groups, queue, s);
 }
 
 }  // namespace czw

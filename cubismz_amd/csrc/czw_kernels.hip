@@ -41,53 +41,6 @@ constexpr size_t SLOT32 = MASK_BYTES + 4 * (size_t)NB3;
 
 
 // ---------------------------------------------------------------------------
-// Stream segment g covers stream indices [32g, 32g+32) (g = t + NT*k for the
-// k-th segment of thread t).  Stream order (FullWaveletTransform.h:59-96,
-// 163-176): [0,512) is the [0,8)^3 corner as codes 0..7 of boxes of h=4
-// (code 0 = the untouched 4^3 base), [512,4096) codes 1..7 of h=8,
-// [4096,32768) codes 1..7 of h=16.  Inside a code box (origin
-// h*(code&1, code>>1&1, code>>2&1) on (a2,a1,a0)) the order is a0, a1, a2
-// with a2 fastest, so a segment is 32/h runs of h consecutive a2.  For each
-// run the callback receives the run index q, the run length h, the LDS address
-// of its first element, the LDS stride along the run, the mask word p>>5 and
-// the bit shift of its first element.
-template <typename F>
-CZW_DI void for_each_run(int g, F &&f)
-{
-    if (g < 16) {  // h = 4: coarse base + level-8 details, O = (a0,a1,a2)
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int s = 32 * g + 4 * q;
-            const int code = s >> 6, k = s & 63;
-            const int a0 = 4 * ((code >> 2) & 1) + (k >> 4);
-            const int a1 = 4 * ((code >> 1) & 1) + ((k >> 2) & 3);
-            const int o2 = 4 * (code & 1);
-            f(q, 4, a0 * SP + a1 * RP + o2, 1, a1 + 32 * a0, o2);
-        }
-    } else if (g < 128) {  // h = 8: level-16 details, O.x=a0, O.z=a1, O.y=a2
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int s = 32 * g + 8 * q;
-            const int code = s >> 9, k = s & 511;
-            const int a0 = 8 * ((code >> 2) & 1) + (k >> 6);
-            const int a1 = 8 * ((code >> 1) & 1) + ((k >> 3) & 7);
-            const int o2 = 8 * (code & 1);
-            f(q, 8, a1 * SP + o2 * RP + a0, RP, a1 + 32 * a0, o2);
-        }
-    } else {  // h = 16: level-32 details, O.y=a0, O.x=a1, O.z=a2
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int s = 32 * g + 16 * q;
-            const int code = s >> 12, k = s & 4095;
-            const int a0 = 16 * ((code >> 2) & 1) + (k >> 8);
-            const int a1 = 16 * ((code >> 1) & 1) + ((k >> 4) & 15);
-            const int o2 = 16 * (code & 1);
-            f(q, 16, o2 * SP + a0 * RP + a1, SP, a1 + 32 * a0, o2);
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // One level-32 pass over this thread's NROW rows (row k starts at base[k],
 // elements `stride` apart): every row is loaded before the first one is
 // written back, so the loads of the second row overlap the arithmetic of the
@@ -112,33 +65,6 @@ CZW_DI void ypass_bases(int t, int (&base)[NROW])
 #pragma unroll
     for (int k = 0; k < NROW; ++k) base[k] = ((t + NT * k) >> 5) * SP + ((t + NT * k) & 31);
 }
-
-template <int WT>
-CZW_DI void iwt_most_f32(float *A, int t)
-{
-    iwt16_8<WT, RP, SP>(A, t);
-    {  // level 16 z-pass, then level 32 z-pass: rows (y, x)
-        float r[NROW][32];
-#pragma unroll
-        for (int k = 0; k < NROW; ++k) lds_ld<32>(A, ((t + NT * k) >> 5) * RP + ((t + NT * k) & 31), SP, r[k]);
-#pragma unroll
-        for (int k = 0; k < NROW; ++k) {
-            const int q = t + NT * k, y = q >> 5, x = q & 31;
-            if (y < 16 && x < 16) row<16, false, WT>(r[k]);
-            row<32, false, WT>(r[k]);
-            lds_st<32>(A, y * RP + x, SP, r[k]);
-        }
-    }
-    __syncthreads();
-    {  // level 32 y-pass
-        int base[NROW];
-        ypass_bases(t, base);
-        pass_rows_f32<false, WT>(A, base, RP);
-    }
-    __syncthreads();
-}
-
-
 
 // ---------------------------------------------------------------------------
 // x-row q (0..1023) of a block: in the block layout q = (z, y) with y
@@ -179,21 +105,6 @@ CZW_DI void load_xrow_f32(const float *__restrict__ src, const FieldGeom &g, int
     }
 }
 
-template <int LAYOUT>
-CZW_DI void store_xrow_f32(float *__restrict__ dst, const FieldGeom &g, int b, int q, const float *r)
-{
-    if constexpr (LAYOUT == LAYOUT_BLOCKS) {
-        float4 *p = reinterpret_cast<float4 *>(dst + (size_t)b * NB3 + (size_t)q * 32);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) p[i] = make_float4(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
-    } else {
-        size_t xs;
-        float *p = dst + field_row(g, b, q >> 5, q & 31, &xs);
-#pragma unroll
-        for (int x = 0; x < 32; ++x) p[x * xs] = r[x];
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Compressor, binary32.  Mirrors WaveletCompressorGeneric::compress
 // (WaveletCompressor.cpp:153-182): fwt, threshold + mask, zerobits, shuffle.
@@ -201,23 +112,6 @@ CZW_DI void store_xrow_f32(float *__restrict__ dst, const FieldGeom &g, int b, i
 // and loads block b+G into registers while block b is transformed, so the
 // HBM reads of one block overlap the arithmetic of the previous one.  Block
 // b's payload goes to slots + b*SLOT32 and its size to nbytes[b].
-// LDS address of bit j (0..31) of stream segment g (see for_each_run).
-CZW_DI int seg_lds_addr(int g, int j)
-{
-    if (g < 16) {
-        const int s = 32 * g + j, code = s >> 6, k = s & 63;
-        const int a0 = 4 * ((code >> 2) & 1) + (k >> 4), a1 = 4 * ((code >> 1) & 1) + ((k >> 2) & 3);
-        return a0 * SP + a1 * RP + 4 * (code & 1) + (k & 3);
-    } else if (g < 128) {
-        const int s = 32 * g + j, code = s >> 9, k = s & 511;
-        const int a0 = 8 * ((code >> 2) & 1) + (k >> 6), a1 = 8 * ((code >> 1) & 1) + ((k >> 3) & 7);
-        return a1 * SP + (8 * (code & 1) + (k & 7)) * RP + a0;
-    } else {
-        const int s = 32 * g + j, code = s >> 12, k = s & 4095;
-        const int a0 = 16 * ((code >> 2) & 1) + (k >> 8), a1 = 16 * ((code >> 1) & 1) + ((k >> 4) & 15);
-        return (16 * (code & 1) + (k & 15)) * SP + a0 * RP + a1;
-    }
-}
 
 // Slot layout written by the compressor (block b at slots + b*SLOT32):
 //   [mask 4096 B][survivors of the [0,16)^3 corner, stream order, <= 4096
@@ -227,7 +121,6 @@ constexpr int CORNER_MAX = 4096;
 constexpr int NRUN32 = 1792;  // level-32 stream runs: 7 codes x 16 x 16 runs of 16
 constexpr int STAGE32_AT = 16 * SP;                      // the z >= 16 half of A
 constexpr int STAGE32 = ((LDS_WORDS - STAGE32_AT - 64) / 4) * 4;  // staged survivors (64 dummy words follow)
-
 
 //
 // Block order: workgroup w starts with block w; the next block is w + G
@@ -512,374 +405,6 @@ __global__ __launch_bounds__(256) void k_pack_records(const uint8_t *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Decompressor, binary32.  Mirrors WaveletCompressorGeneric::decompress
-// (WaveletCompressor.cpp:232-259): mask, reshuffle, load (stream order,
-// zeros for dropped coefficients; FullWaveletTransform.h:100-150), iwt.
-// Persistent like the compressor: while block b is decoded, the record of
-// block b+G (size word, mask, first PF*NT survivor words) is already in
-// flight and offsets[b+2G] is being fetched.
-constexpr int PF = 8;
-
-// Survivor words sv[k] = survivor k*NT + t were loaded for the indices below
-// `lim`; the record's extent is taken from the next record's offset when the
-// records are in order (a hint only: survivors past lim are read later), so
-// the prefetch does not read into the following records.
-struct RecPrefetch {
-    uint32_t nb, lim, mw[NROW], sv[PF];
-};
-
-CZW_DI void fetch_record(const uint8_t *__restrict__ records, uint64_t rec_bytes, uint64_t ob, uint64_t ob_end,
-                         int t, RecPrefetch &r)
-{
-    if ((ob & 3u) || ob + 4 + MASK_BYTES > rec_bytes) {
-        r.nb = 0xffffffffu;  // rejected by the size check
-        r.lim = 0u;
-#pragma unroll
-        for (int k = 0; k < NROW; ++k) r.mw[k] = 0u;
-#pragma unroll
-        for (int k = 0; k < PF; ++k) r.sv[k] = 0u;
-        return;
-    }
-    const uint32_t *p = reinterpret_cast<const uint32_t *>(records + ob);
-    uint64_t nwords = (rec_bytes - ob) / 4;  // words of the buffer from this record on
-    if (ob_end > ob + 4 + MASK_BYTES && (ob_end - ob) / 4 < nwords) nwords = (ob_end - ob) / 4;
-    r.lim = nwords > 1 + 1024 ? (uint32_t)min(nwords - 1 - 1024, (uint64_t)(PF * NT)) : 0u;
-    r.nb = p[0];
-#pragma unroll
-    for (int k = 0; k < NROW; ++k) r.mw[k] = p[1 + t + NT * k];
-#pragma unroll
-    for (int k = 0; k < PF; ++k) {
-        const uint32_t i = (uint32_t)(k * NT + t);
-        r.sv[k] = i < r.lim ? p[1 + 1024 + i] : 0u;
-    }
-}
-
-// Undo the byte-plane shuffle of S staged survivors in place (byte p of
-// survivor i at byte p*S + i, WaveletCompressor.cpp:75-93): each thread
-// gathers its survivors i = t + NT*j into registers, then all write back.
-template <int MAXPT>
-CZW_DI void unshuffle_stage(uint32_t *stage, uint32_t S)
-{
-    const int t = threadIdx.x;
-    const uint8_t *b8 = reinterpret_cast<const uint8_t *>(stage);
-    uint32_t u[MAXPT];
-#pragma unroll
-    for (int j = 0; j < MAXPT; ++j) {
-        const uint32_t i = (uint32_t)(t + NT * j);
-        u[j] = i < S ? ((uint32_t)b8[i] | ((uint32_t)b8[S + i] << 8) | ((uint32_t)b8[2 * S + i] << 16) |
-                        ((uint32_t)b8[3 * S + i] << 24))
-                     : 0u;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < MAXPT; ++j) {
-        const uint32_t i = (uint32_t)(t + NT * j);
-        if (i < S) stage[i] = u[j];
-    }
-    __syncthreads();
-}
-
-// Decompression of one block whose survivors fit the z >= 16 half of A
-// (S <= STAGE32): survivors are staged there; the corner positions are
-// written into A branch-free (value or +0); the level-32 survivors are
-// gathered straight into registers for the fused level-16/level-32 z-pass.
-// Returns false (uniformly) when the record is inconsistent.
-template <int WT>
-CZW_DI bool decode_block_fast_f32(float *A, uint32_t *M, uint32_t *cnt, uint32_t *wsum, uint32_t *flag,
-                                  const RecPrefetch &cur, const uint32_t *pay, uint32_t S, int shuffle,
-                                  float (&r)[NROW][32])
-{
-    const int t = threadIdx.x;
-    uint32_t *stage = reinterpret_cast<uint32_t *>(A) + STAGE32_AT;
-    const uint8_t *stage8 = reinterpret_cast<const uint8_t *>(stage);
-    if (t == 0) *flag = 1u;
-#pragma unroll
-    for (int k = 0; k < NROW; ++k) M[t + NT * k] = cur.mw[k];
-#pragma unroll
-    for (int k = 0; k < PF; ++k) {
-        const uint32_t i = (uint32_t)(k * NT + t);
-        if (i < S) stage[i] = i < cur.lim ? cur.sv[k] : pay[MASK_BYTES / 4 + i];
-    }
-    for (uint32_t i = PF * NT + t; i < S; i += NT) stage[i] = pay[MASK_BYTES / 4 + i];
-    if (t < 64) stage[STAGE32 + t] = 0u;  // the dummy words read for dropped coefficients
-    __syncthreads();
-    if (shuffle) unshuffle_stage<(STAGE32 + NT - 1) / NT>(stage, S);
-    // the 4^3 base is always stored (FullWaveletTransform.h:163-176): its 64
-    // mask bits are the low nibbles of words a1 + 32*a0, a0, a1 < 4
-    if (t < 16 && (M[(t & 3) + 32 * (t >> 2)] & 0xfu) != 0xfu) *flag = 0u;
-    // counts in stream order: corner piece t at cnt[t], level-32 run rho at cnt[512 + rho]
-    uint32_t keep = 0;
-    corner_piece<RP, SP>(t, [&](int q, int h, int, int, int word, int shift) {
-        const uint32_t hm = (1u << h) - 1u;
-        keep |= ((t < 8) ? hm : ((M[word] >> shift) & hm)) << (q * h);
-    });
-    cnt[t] = __popc(keep);
-    uint32_t mw[NROW];
-#pragma unroll
-    for (int k = 0; k < NROW; ++k) {
-        const int q = t + NT * k, y = q >> 5, x = q & 31;
-        const bool corner = (y < 16) && (x < 16);
-        mw[k] = M[x + 32 * y] & (corner ? 0xffff0000u : 0xffffffffu);
-        const int c0 = 2 * (x >> 4) + 4 * (y >> 4), base = (y & 15) * 16 + (x & 15);
-        if (!corner) cnt[512 + (c0 - 1) * 256 + base] = __popc(mw[k] & 0xffffu);
-        cnt[512 + c0 * 256 + base] = __popc(mw[k] >> 16);
-    }
-    if (t < 2560 - 512 - NRUN32) cnt[512 + NRUN32 + t] = 0u;
-    __syncthreads();
-    {  // exclusive scan of the 2560 counts, 5 consecutive per thread
-        uint32_t c5[5], sum = 0;
-#pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            c5[i] = cnt[5 * t + i];
-            sum += c5[i];
-        }
-        uint32_t tot;
-        uint32_t run = block_excl_scan(sum, wsum, &tot);
-#pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            cnt[5 * t + i] = run;
-            run += c5[i];
-        }
-        if (tot != S) *flag = 0u;
-    }
-    __syncthreads();
-    if (!*flag) return false;
-    // the 64 dummy words after the stage hold +0: a dropped coefficient reads
-    // its lane's dummy word
-    auto sv = [&](uint32_t i) -> float { return __uint_as_float(stage[i]); };
-    const uint32_t dummy = (uint32_t)STAGE32 + (t & 63);
-    {  // corner positions: survivor or +0
-        uint32_t idx = cnt[t];
-        corner_piece<RP, SP>(t, [&](int q, int h, int addr, int stride, int, int) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                if (i < h) {
-                    const uint32_t bit = (keep >> (q * h + i)) & 1u;
-                    const float v = sv(bit ? idx : dummy);
-                    A[addr + i * stride] = v;
-                    idx += bit;
-                }
-            }
-        });
-    }
-    // level-32 survivors of this thread's z-columns, straight into registers
-#pragma unroll
-    for (int k = 0; k < NROW; ++k) {
-        const int q = t + NT * k, y = q >> 5, x = q & 31;
-        const int c0 = 2 * (x >> 4) + 4 * (y >> 4), base = (y & 15) * 16 + (x & 15);
-        uint32_t i0 = (c0 > 0) ? cnt[512 + (c0 - 1) * 256 + base] : 0u;
-        uint32_t i1 = cnt[512 + c0 * 256 + base];
-#pragma unroll
-        for (int z = 0; z < 32; ++z) {
-            const uint32_t bit = (mw[k] >> z) & 1u;
-            uint32_t &i = z < 16 ? i0 : i1;
-            const float v = sv(bit ? i : dummy);
-            r[k][z] = v;
-            i += bit;
-        }
-    }
-    __syncthreads();  // stage consumed; corner of A complete
-    return true;
-}
-
-//
-// Block order as in the compressor, three deep: the record of the next block
-// b1 and the offset of the one after, b2, are in flight while b is decoded.
-// Workgroup w starts with w, w + G, w + 2G; then b2 advances by G (static) or
-// takes the ticket 3G + atomicAdd(queue, 1) of a cleared work queue.
-template <int WT, int LAYOUT>
-__global__ __launch_bounds__(NT) void k_decompress_f32(const uint8_t *__restrict__ records, uint64_t rec_bytes,
-                                                       const uint64_t *__restrict__ offsets, int nblocks,
-                                                       int shuffle, float *__restrict__ dst, FieldGeom g,
-                                                       uint32_t *__restrict__ err,
-                                                       const czw_field_t *__restrict__ fields, int nf,
-                                                       uint32_t *__restrict__ queue)
-{
-    constexpr int RL = row_layout<LAYOUT>();
-    __shared__ __align__(16) float A[LDS_WORDS];
-    __shared__ uint32_t M[1024];
-    __shared__ uint32_t cnt[2560];
-    __shared__ uint32_t wsum[NT / 64];
-    __shared__ uint32_t flag;
-    __shared__ int ticket[2];
-    __shared__ czw_field_t T[LAYOUT == LAYOUT_BATCH ? BATCH_MAX_FIELDS : 1];
-    const int t = threadIdx.x;
-    const int G = gridDim.x;
-    load_table<LAYOUT>(T, fields, nf);
-    // offset of a block's record and the next block's (an extent hint; 0 = none)
-    auto off_at = [&](int i) -> uint64_t { return i < nblocks ? offsets[i] : 0; };
-    int b = blockIdx.x, b1 = b + G, b2 = b + 2 * G, bq = 0;
-    uint64_t ob0 = off_at(b);                          // block b
-    uint64_t ob1 = off_at(b1), ob1_end = off_at(b1 + 1);  // block b1
-    RecPrefetch pf;
-    if (b < nblocks) fetch_record(records, rec_bytes, ob0, off_at(b + 1), t, pf);
-    for (int it = 0; b < nblocks; ++it, b = b1, b1 = b2, b2 = bq) {
-        if (queue) {
-            if (t == 0) ticket[it & 1] = 3 * G + (int)atomicAdd(queue, 1u);
-            __syncthreads();
-            bq = ticket[it & 1];
-        } else {
-            bq = b2 + G;
-        }
-        const RecPrefetch cur = pf;
-        const uint64_t ob_cur = ob0;
-        if (b1 < nblocks) fetch_record(records, rec_bytes, ob1, ob1_end, t, pf);
-        ob0 = ob1;
-        if (b2 < nblocks) {
-            ob1 = offsets[b2];
-            ob1_end = off_at(b2 + 1);
-        }
-
-        const uint32_t nb = cur.nb;
-        if (nb < MASK_BYTES || ((nb - MASK_BYTES) & 3u) || nb > SLOT32 || ob_cur + 4 + nb > rec_bytes) {
-            if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_NBYTES);
-            continue;
-        }
-        const uint32_t S = (nb - MASK_BYTES) >> 2;
-        const uint32_t *pay = reinterpret_cast<const uint32_t *>(records + ob_cur + 4);
-        __syncthreads();  // the previous block's last reads of A, M, cnt and flag are done
-        float r[NROW][32];
-        if (S <= (uint32_t)STAGE32) {
-            if (!decode_block_fast_f32<WT>(A, M, cnt, wsum, &flag, cur, pay, S, shuffle, r)) {
-                if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_POPCOUNT);
-                continue;
-            }
-            iwt16_8<WT, RP, SP>(A, t);
-            // level 16 z-pass (corner columns), then level 32 z-pass; both
-            // threads' corner loads are issued before the first store
-#pragma unroll
-            for (int k = 0; k < NROW; ++k) {
-                const int q = t + NT * k, y = q >> 5, x = q & 31;
-                if (y < 16 && x < 16) lds_ld<16>(A, y * RP + x, SP, r[k]);
-            }
-#pragma unroll
-            for (int k = 0; k < NROW; ++k) {
-                const int q = t + NT * k, y = q >> 5, x = q & 31;
-                if (y < 16 && x < 16) row<16, false, WT>(r[k]);
-                row<32, false, WT>(r[k]);
-                lds_st<32>(A, y * RP + x, SP, r[k]);
-            }
-            __syncthreads();
-        } else {
-            // dense block: survivors staged over A, every position gathered
-            // into registers, then placed; the full inverse follows
-            if (t == 0) flag = 1u;
-#pragma unroll
-            for (int k = 0; k < NROW; ++k) M[t + NT * k] = cur.mw[k];
-            uint32_t *stage = reinterpret_cast<uint32_t *>(A);
-#pragma unroll
-            for (int k = 0; k < PF; ++k) {
-                const uint32_t i = (uint32_t)(k * NT + t);  // < S: dense blocks hold more than PF*NT survivors
-                stage[i] = i < cur.lim ? cur.sv[k] : pay[MASK_BYTES / 4 + i];
-            }
-            for (uint32_t i = PF * NT + t; i < S; i += NT) stage[i] = pay[MASK_BYTES / 4 + i];
-            __syncthreads();
-            if (t < 16 && (M[(t & 3) + 32 * (t >> 2)] & 0xfu) != 0xfu) flag = 0u;
-            uint32_t keep[NROW];
-#pragma unroll
-            for (int k = 0; k < NROW; ++k) {
-                const int sg = t + NT * k;
-                keep[k] = 0;
-                for_each_run(sg, [&](int q, int h, int, int, int word, int shift) {
-                    const uint32_t hm = (h == 16) ? 0xffffu : ((1u << h) - 1u);
-                    const uint32_t bits = (sg < 2) ? hm : ((M[word] >> shift) & hm);
-                    keep[k] |= bits << (q * h);
-                });
-            }
-            uint32_t tot;
-            const uint32_t ex =
-                block_excl_scan((uint32_t)__popc(keep[0]) | ((uint32_t)__popc(keep[1]) << 16), wsum, &tot);
-            // with all base bits set, the kept count equals the mask popcount
-            // that WaveletCompressor.cpp:235-251 checks against the payload size
-            if ((tot & 0xffffu) + (tot >> 16) != S || !flag) {
-                if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_POPCOUNT);
-                continue;
-            }
-            const uint32_t first[NROW] = {ex & 0xffffu, (tot & 0xffffu) + (ex >> 16)};
-#pragma unroll
-            for (int k = 0; k < NROW; ++k) {
-                uint32_t idx = first[k];
-#pragma unroll
-                for (int j = 0; j < 32; ++j) {
-                    float c = 0.0f;
-                    if ((keep[k] >> j) & 1u) {
-                        const uint8_t *b8 = reinterpret_cast<const uint8_t *>(stage);
-                        c = __uint_as_float(shuffle ? ((uint32_t)b8[idx] | ((uint32_t)b8[S + idx] << 8) |
-                                                       ((uint32_t)b8[2 * S + idx] << 16) |
-                                                       ((uint32_t)b8[3 * S + idx] << 24))
-                                                    : stage[idx]);
-                        ++idx;
-                    }
-                    r[k][j] = c;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < NROW; ++k) {
-                for_each_run(t + NT * k, [&](int q, int h, int addr, int stride, int, int) {
-#pragma unroll
-                    for (int i = 0; i < 16; ++i)
-                        if (i < h) A[addr + i * stride] = r[k][q * h + i];
-                });
-            }
-            __syncthreads();
-            iwt_most_f32<WT>(A, t);
-        }
-        if (S <= (uint32_t)STAGE32) {  // level 32 y-pass
-            int base[NROW];
-            ypass_bases(t, base);
-            pass_rows_f32<false, WT>(A, base, RP);
-            __syncthreads();
-        }
-        if constexpr (RL == LAYOUT_BLOCKS) {
-#pragma unroll
-            for (int k = 0; k < NROW; ++k) {
-                float rr[32];
-                lds_ld<32>(A, xrow_lds_base<RL>(t + NT * k), 1, rr);
-                row<32, false, WT>(rr);  // level 32, x-pass
-                store_xrow_f32<RL>(dst, g, b, t + NT * k, rr);
-            }
-        } else {
-            // level 32 x-pass back into A, then 16-byte stores along the
-            // field's contiguous Z: four times fewer store instructions than
-            // storing each x-row element on its own
-            {
-                int base[NROW];
-#pragma unroll
-                for (int k = 0; k < NROW; ++k) base[k] = (t & 31) * SP + ((t + NT * k) >> 5) * RP;
-                pass_rows_f32<false, WT>(A, base, 1);
-            }
-            __syncthreads();
-            const BlockRef<float> br = locate<LAYOUT, float>(T, nf, b, nullptr, dst, g, 0.0f);
-            size_t xs;
-            float *p0 = br.dst + field_row(br.g, br.lb, 0, 0, &xs);
-            const int nz = br.g.nz;
-            if constexpr (RL == LAYOUT_FIELD) {
-                // x = i >> 8 is the same for the 64 lanes of a wave: a scalar offset
-                const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(p0);
-                const int so = (int)xs * 4;
-#pragma unroll 4
-                for (int i = t; i < 32 * 32 * 8; i += NT) {  // (x, y, z4) with z4 fastest
-                    const int z4 = i & 7, y = (i >> 3) & 31, x = __builtin_amdgcn_readfirstlane(i >> 8);
-                    const int a = (4 * z4) * SP + y * RP + x;
-                    const czw_u32x4 v = {__float_as_uint(A[a]), __float_as_uint(A[a + SP]), __float_as_uint(A[a + 2 * SP]),
-                                         __float_as_uint(A[a + 3 * SP])};
-                    __builtin_amdgcn_raw_buffer_store_b128(v, rs, (y * nz + 4 * z4) * 4, x * so, 0);
-                }
-            } else {
-#pragma unroll 4
-                for (int i = t; i < 32 * 32 * 8; i += NT) {  // (x, y, z4) with z4 fastest
-                    const int z4 = i & 7, y = (i >> 3) & 31, x = i >> 8;
-                    const int a = (4 * z4) * SP + y * RP + x;
-                    const float4 v = make_float4(A[a], A[a + SP], A[a + 2 * SP], A[a + 3 * SP]);
-                    *reinterpret_cast<float4 *>(p0 + x * xs + (size_t)y * nz + 4 * z4) = v;
-                }
-            }
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // host-side dispatch
 
 
@@ -935,23 +460,6 @@ static int compress_f64(int layout, const void *src, int nblocks, FieldGeom g, f
     return pack_records_f64(slots, nb, nblocks, shuffle, static_cast<uint8_t *>(records), offsets, s);
 }
 
-template <int LAYOUT>
-static int decompress_f32(const void *records, size_t rec_bytes, const uint64_t *offsets, int nblocks, FieldGeom g,
-                          int wtype, int shuffle, float *dst, uint32_t *err, hipStream_t s)
-{
-    const int wt = wt_index(wtype);
-    if (wt < 0 || nblocks <= 0 || !records || !offsets || !dst || !err) return CZW_EINVAL;
-    const uint8_t *rec = static_cast<const uint8_t *>(records);
-    const void *fn = wt == 1 ? (const void *)k_decompress_f32<1, LAYOUT>
-                   : wt == 2 ? (const void *)k_decompress_f32<2, LAYOUT> : (const void *)k_decompress_f32<3, LAYOUT>;
-    const dim3 grid(std::min(nblocks, resident_groups(fn, NT))), blk(NT);
-    const czw_field_t *nf0 = nullptr;
-    uint32_t *nq = nullptr;
-    if (wt == 1) hipLaunchKernelGGL((k_decompress_f32<1, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, nblocks, shuffle, dst, g, err, nf0, 0, nq);
-    if (wt == 2) hipLaunchKernelGGL((k_decompress_f32<2, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, nblocks, shuffle, dst, g, err, nf0, 0, nq);
-    if (wt == 3) hipLaunchKernelGGL((k_decompress_f32<3, LAYOUT>), grid, blk, 0, s, rec, (uint64_t)rec_bytes, offsets, nblocks, shuffle, dst, g, err, nf0, 0, nq);
-    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
-}
 
 
 // ---------------------------------------------------------------------------
@@ -1013,25 +521,12 @@ static int batch_decompress(const czw_batch_t *B, hipStream_t s)
 {
     if (!batch_ok(B) || !B->err) return CZW_EINVAL;
     if (B->prec != 4 && B->prec != 8) return CZW_EINVAL;
-    const int wt = wt_index(B->wtype);
     uint32_t *q = B->queue ? batch_queue(B) + 1 : nullptr;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, B->err, 1, 0u);
     if (q) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, q, 1, 0u);
-    if (B->prec == 8)
-        return decompress_f64(LAYOUT_BATCH, B->records, B->records_bytes, B->offsets, B->nblocks, FieldGeom{0, 0, 0},
-                              B->wtype, B->shuffle, nullptr, B->err, B->fields, B->nfields, B->groups, q, s);
-    const uint8_t *rec = static_cast<const uint8_t *>(B->records);
-    const void *fn = wt == 1 ? (const void *)k_decompress_f32<1, LAYOUT_BATCH>
-                   : wt == 2 ? (const void *)k_decompress_f32<2, LAYOUT_BATCH> : (const void *)k_decompress_f32<3, LAYOUT_BATCH>;
-    const int groups = B->groups ? B->groups : resident_groups(fn, NT);
-    const dim3 grid(std::min(B->nblocks, groups)), blk(NT);
-    float *dst0 = nullptr;
-    const FieldGeom g{0, 0, 0};
-    const uint64_t rb = (uint64_t)B->records_bytes;
-    if (wt == 1) hipLaunchKernelGGL((k_decompress_f32<1, LAYOUT_BATCH>), grid, blk, 0, s, rec, rb, B->offsets, B->nblocks, B->shuffle, dst0, g, B->err, B->fields, B->nfields, q);
-    if (wt == 2) hipLaunchKernelGGL((k_decompress_f32<2, LAYOUT_BATCH>), grid, blk, 0, s, rec, rb, B->offsets, B->nblocks, B->shuffle, dst0, g, B->err, B->fields, B->nfields, q);
-    if (wt == 3) hipLaunchKernelGGL((k_decompress_f32<3, LAYOUT_BATCH>), grid, blk, 0, s, rec, rb, B->offsets, B->nblocks, B->shuffle, dst0, g, B->err, B->fields, B->nfields, q);
-    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
+    return decompress_half(B->prec, LAYOUT_BATCH, B->records, B->records_bytes, B->offsets, B->nblocks,
+                           FieldGeom{0, 0, 0}, B->wtype, B->shuffle, nullptr, B->err, B->fields, B->nfields, B->groups,
+                           q, s);
 }
 
 } // namespace czw
@@ -1143,28 +638,18 @@ int czw_decompress_field(const void *records, size_t records_bytes, const uint64
     const int nb = (nx / 32) * (ny / 32) * (nz / 32);
     const FieldGeom g{nx, ny, nz};
     const bool buf = field_fits_buffer(ny, nz, prec);
-    if (prec == 4)
-        return buf ? decompress_f32<LAYOUT_FIELD>(records, records_bytes, offsets, nb, g, wtype, shuffle,
-                                                  static_cast<float *>(field), err, stream)
-                   : decompress_f32<LAYOUT_FIELD_WIDE>(records, records_bytes, offsets, nb, g, wtype, shuffle,
-                                                       static_cast<float *>(field), err, stream);
-    if (prec == 8)
-        return decompress_f64(buf ? LAYOUT_FIELD : LAYOUT_FIELD_WIDE, records, records_bytes, offsets, nb, g, wtype,
-                              shuffle, field, err, nullptr, 0, 0, nullptr, stream);
-    return CZW_EINVAL;
+    if (prec != 4 && prec != 8) return CZW_EINVAL;
+    return decompress_half(prec, buf ? LAYOUT_FIELD : LAYOUT_FIELD_WIDE, records, records_bytes, offsets, nb, g,
+                           wtype, shuffle, field, err, nullptr, 0, 0, nullptr, stream);
 }
 
 int czw_decompress_blocks(const void *records, size_t records_bytes, const uint64_t *offsets, int nblocks, int prec,
                           int wtype, int shuffle, void *blocks, uint32_t *err, hipStream_t stream)
 {
     const FieldGeom g{0, 0, 0};
-    if (prec == 4)
-        return decompress_f32<LAYOUT_BLOCKS>(records, records_bytes, offsets, nblocks, g, wtype, shuffle,
-                                             static_cast<float *>(blocks), err, stream);
-    if (prec == 8)
-        return decompress_f64(LAYOUT_BLOCKS, records, records_bytes, offsets, nblocks, g, wtype, shuffle, blocks,
-                              err, nullptr, 0, 0, nullptr, stream);
-    return CZW_EINVAL;
+    if (prec != 4 && prec != 8) return CZW_EINVAL;
+    return decompress_half(prec, LAYOUT_BLOCKS, records, records_bytes, offsets, nblocks, g, wtype, shuffle, blocks,
+                           err, nullptr, 0, 0, nullptr, stream);
 }
 
 int czw_batch_prepare(czw_field_t *fields, int nfields, int prec, int *nblocks)

@@ -147,7 +147,7 @@ typedef struct czw_field {
     int32_t nblocks; /* set by czw_batch_prepare */
 } czw_field_t;
 
-#define CZW_BATCH_MAX_FIELDS 128
+#define CZW_BATCH_MAX_FIELDS 64
 
 typedef struct czw_batch {
     const czw_field_t *fields; /* DEVICE copy of the prepared table */

@@ -43,5 +43,5 @@ def test_batch_prepare_host_only():
     with pytest.raises(czw.CZWError):
         B.prepare_table([(48, 32, 32)], [1e-3])
     with pytest.raises(czw.CZWError):
-        B.prepare_table([(32, 32, 32)] * 129, [1e-3] * 129)
+        B.prepare_table([(32, 32, 32)] * 65, [1e-3] * 65)
     assert czw.load_library().czw_version() >= 100
