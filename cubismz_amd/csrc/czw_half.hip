@@ -147,6 +147,21 @@ CZW_DI float zero_low_bits(float v, uint32_t zkeep) { return __uint_as_float(__f
 CZW_DI float absval(float v) { return fabsf(v); }
 CZW_DI double absval(double v) { return fabs(v); }
 
+// Stage of the x < 16 half's survivors: the z >= 16 planes of H.
+constexpr int STAGE0_AT = 16 * SPD;
+constexpr int STAGE0 = HALF_WORDS - STAGE0_AT;
+
+// n staged values (LDS, 16-byte aligned) to global memory (16-byte aligned),
+// 16 bytes per store where possible.
+template <typename R>
+CZW_DI void copy_out(R *__restrict__ dst, const R *src, uint32_t n, int t)
+{
+    constexpr uint32_t V = 16 / sizeof(R);
+    const uint32_t nv = n / V;
+    for (uint32_t i = t; i < nv; i += NT) reinterpret_cast<uint4 *>(dst)[i] = reinterpret_cast<const uint4 *>(src)[i];
+    for (uint32_t i = nv * V + t; i < n; i += NT) dst[i] = src[i];
+}
+
 // Exclusive scan of cnt[0..1024) in place (two entries per thread); returns
 // the total.  Barriers before and after are the caller's.
 CZW_DI uint32_t scan1024(uint32_t *cnt, uint32_t *wsum, int t)
@@ -172,7 +187,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_compress_h(const
                                                      uint32_t *__restrict__ queue)
 {
     constexpr int RL = row_layout<LAYOUT>();
-    __shared__ R H[HALF_WORDS];
+    __shared__ __align__(16) R H[HALF_WORDS];
     __shared__ uint32_t M[1024];
     __shared__ uint32_t cnt[1024];
     __shared__ uint32_t wsum[NT / 64];
@@ -231,13 +246,17 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_compress_h(const
                 cb[6] = cnt[768] - cnt[512];
                 cb[7] = S1 - cnt[768];
             }
+            // every column is in registers: H becomes the stage of this half's
+            // survivors (at most 16384), copied out with 16-byte stores
             uint32_t i0 = cnt[ri * 256 + base], i1 = cnt[(ri + 1) * 256 + base];
 #pragma unroll
             for (int z = 0; z < 32; ++z) {
                 uint32_t &i = z < 16 ? i0 : i1;
-                if ((bits >> z) & 1u) reg[REG_H1 + i++] = zero_low_bits(r[z], zkeep);
+                if ((bits >> z) & 1u) H[i++] = zero_low_bits(r[z], zkeep);
             }
         }
+        __syncthreads();
+        copy_out<R>(reg + REG_H1, H, S1, t);
         __syncthreads();  // H and cnt are free again
 
         // ---- x < 16: level-32 y- and z-passes, level-16 z-pass on the corner columns
@@ -288,15 +307,19 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_compress_h(const
                 cb[4] = cnt[512] - cnt[256];
                 cb[5] = S0 - cnt[512];
             }
+            // the z >= 16 planes of H are free (the corner needs z < 16 only): they
+            // stage this half's survivors when they fit, else these go out directly
             uint32_t i0 = corner ? 0u : cnt[256 + base], i1 = corner ? cnt[base] : cnt[512 + base];
+            R *st = S0 <= (uint32_t)STAGE0 ? H + STAGE0_AT : reg + REG_H0;
 #pragma unroll
             for (int z = 0; z < 32; ++z) {
                 uint32_t &i = z < 16 ? i0 : i1;
-                if ((bits >> z) & 1u) reg[REG_H0 + i++] = zero_low_bits(r[z], zkeep);
+                if ((bits >> z) & 1u) st[i++] = zero_low_bits(r[z], zkeep);
             }
         }
         // levels 16 and 8 on the corner (the barrier inside orders the column stores above)
         __syncthreads();
+        if (S0 <= (uint32_t)STAGE0) copy_out<R>(reg + REG_H0, H + STAGE0_AT, S0, t);
         fwt16_8<WT, RPD, SPD>(H, t);
 
         // the [0,16)^3 corner: stream positions [0,4096) in pieces of 8 (base 4^3 always kept)
