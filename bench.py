@@ -60,6 +60,24 @@ def parse():
     return ap.parse_args()
 
 
+PROFILE = os.path.join(ROOT, "profiles", "czw_f32_field512_eps1e-3_wtype1.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the headline workload, from the
+    calibrated FETCH_SIZE/WRITE_SIZE passes recorded in the rocprof summary."""
+    if not os.path.exists(PROFILE):
+        return None
+    for line in open(PROFILE):
+        if line.startswith("traffic_bytes_per_launch:"):
+            table = json.loads(line.split(":", 1)[1])
+            key = kernel.split("<")[0]
+            for name, v in table.items():
+                if key in name and (("<float" in name) == ("<float" in kernel)):
+                    return int(v)
+    return None
+
+
 def _events(torch, n):
     return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
 
@@ -394,9 +412,11 @@ def main():
     # 4 + 4096 + s*S_b per block; decompress reads that and writes the field
     alg_c = field_bytes + payload
     alg_d = field_bytes + payload + 4 * nblocks
-    kern = {"k_compress_f32": (alg_c, t_c), "k_decompress_f32": (alg_d, t_d)}
+    kern = {"k_compress_f32": (alg_c, t_c), "k_decompress_h<float>": (alg_d, t_d)}
     dom = max(kern, key=lambda k: kern[k][1])
     achieved = kern[dom][0] / kern[dom][1]
+    headline = (n == 512 and a.wtype == 1 and a.eps == 1e-3 and a.zerobits == 0 and not a.shuffle and a.bubbles == 200)
+    traffic = profiled_traffic(dom) if headline else None
 
     recs = records[: int(off[-1])].cpu().numpy()
     outh = out.cpu().numpy()
@@ -450,7 +470,7 @@ def main():
                    "eps": a.eps, "wtype": a.wtype, "zerobits": a.zerobits, "shuffle": a.shuffle},
         "stages_ms": {"compress": round(t_c * 1e3, 4), "pack": round(t_p * 1e3, 4), "decompress": round(t_d * 1e3, 4)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "alg_bytes_per_launch": kern[dom][0], "avg_launch_ms": round(kern[dom][1] * 1e3, 4)},
         "step_roofline_frac": round((alg_c + alg_d) / t_step / HBM_PEAK, 4),
         "other_wtypes": other,
