@@ -256,12 +256,16 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
             uint32_t i0 = (c0 > 0) ? cnt[(c0 - 1) * 256 + base] : 0u;
             uint32_t i1 = cnt[c0 * 256 + base];
             if (staged) {
+                // byte addresses into A; zerobits are applied by the copy-out
+                char *Ab = reinterpret_cast<char *>(A);
+                uint32_t a0 = (STAGE32_AT + i0) * 4u, a1 = (STAGE32_AT + i1) * 4u;
+                const uint32_t ad = (STAGE32_AT + dummy) * 4u;
 #pragma unroll
                 for (int z = 0; z < 32; ++z) {
                     const uint32_t bit = (kb[k] >> z) & 1u;
-                    uint32_t &i = z < 16 ? i0 : i1;
-                    st32[bit ? i : dummy] = __float_as_uint(r[k][z]) & zkeep;
-                    i += bit;
+                    uint32_t &a = z < 16 ? a0 : a1;
+                    *reinterpret_cast<float *>(Ab + (bit ? a : ad)) = r[k][z];
+                    a += bit << 2;
                 }
             } else {
 #pragma unroll
@@ -279,8 +283,15 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
         if (staged) {
             const uint4 *s4 = reinterpret_cast<const uint4 *>(st32);
             uint4 *d4 = reinterpret_cast<uint4 *>(d32);
-            for (uint32_t i = t; i < S32 / 4; i += NT) d4[i] = s4[i];
-            for (uint32_t i = (S32 & ~3u) + t; i < S32; i += NT) d32[i] = st32[i];
+            for (uint32_t i = t; i < S32 / 4; i += NT) {
+                uint4 v = s4[i];
+                v.x &= zkeep;
+                v.y &= zkeep;
+                v.z &= zkeep;
+                v.w &= zkeep;
+                d4[i] = v;
+            }
+            for (uint32_t i = (S32 & ~3u) + t; i < S32; i += NT) d32[i] = st32[i] & zkeep;
         }
         fwt16_8<WT, RP, SP>(A, t);
 
