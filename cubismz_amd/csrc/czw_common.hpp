@@ -53,6 +53,7 @@ CZW_DI uint32_t wave_incl_scan(uint32_t x, int lane)
 // Block-wide exclusive scan of one value per thread.  Returns the exclusive
 // prefix; *total receives the sum.  Ends with the wave totals read, so the
 // caller must barrier before wsum is written again.
+template <int NTH = NT>
 CZW_DI uint32_t block_excl_scan(uint32_t v, uint32_t *wsum, uint32_t *total)
 {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -61,7 +62,7 @@ CZW_DI uint32_t block_excl_scan(uint32_t v, uint32_t *wsum, uint32_t *total)
     __syncthreads();
     uint32_t base = 0, tot = 0;
 #pragma unroll
-    for (int i = 0; i < NT / 64; ++i) {
+    for (int i = 0; i < NTH / 64; ++i) {
         const uint32_t s = wsum[i];
         base += (i < w) ? s : 0u;
         tot += s;

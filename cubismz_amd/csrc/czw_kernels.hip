@@ -37,33 +37,36 @@ constexpr int RP = 33;
 constexpr int SP = 32 * 33 + 1;
 constexpr int LDS_WORDS = 32 * SP;
 constexpr size_t SLOT32 = MASK_BYTES + 4 * (size_t)NB3;
+// the compressor's workgroup: CT threads, CR level-32 rows per thread
+constexpr int CT = 512;
+constexpr int CR = 1024 / CT;
 
 
 
 // ---------------------------------------------------------------------------
-// One level-32 pass over this thread's NROW rows (row k starts at base[k],
+// One level-32 pass over this thread's CR rows (row k starts at base[k],
 // elements `stride` apart): every row is loaded before the first one is
 // written back, so the loads of the second row overlap the arithmetic of the
 // first (the compiler cannot hoist them past stores to the same array).
 template <bool FWD, int WT>
-CZW_DI void pass_rows_f32(float *A, const int (&base)[NROW], int stride)
+CZW_DI void pass_rows_f32(float *A, const int (&base)[CR], int stride)
 {
-    float r[NROW][32];
+    float r[CR][32];
 #pragma unroll
-    for (int k = 0; k < NROW; ++k) lds_ld<32>(A, base[k], stride, r[k]);
+    for (int k = 0; k < CR; ++k) lds_ld<32>(A, base[k], stride, r[k]);
 #pragma unroll
-    for (int k = 0; k < NROW; ++k) {
+    for (int k = 0; k < CR; ++k) {
         row<32, FWD, WT>(r[k]);
         lds_st<32>(A, base[k], stride, r[k]);
     }
 }
 
 // Rows of the level-32 passes owned by thread t: y-pass rows (z, x) and
-// z-pass rows (y, x) at q = t + NT*k, x fastest.
-CZW_DI void ypass_bases(int t, int (&base)[NROW])
+// z-pass rows (y, x) at q = t + CT*k, x fastest.
+CZW_DI void ypass_bases(int t, int (&base)[CR])
 {
 #pragma unroll
-    for (int k = 0; k < NROW; ++k) base[k] = ((t + NT * k) >> 5) * SP + ((t + NT * k) & 31);
+    for (int k = 0; k < CR; ++k) base[k] = ((t + CT * k) >> 5) * SP + ((t + CT * k) & 31);
 }
 
 // ---------------------------------------------------------------------------
@@ -128,7 +131,7 @@ constexpr int STAGE32 = ((LDS_WORDS - STAGE32_AT - 64) / 4) * 4;  // staged surv
 // of a work queue whose counter the caller has cleared.  In LAYOUT_BATCH the
 // blocks of nf fields (table `fields`, batch-wide numbering) form one queue.
 template <int WT, int LAYOUT>
-__global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ src, FieldGeom g, int nblocks,
+__global__ __launch_bounds__(CT) void k_compress_f32(const float *__restrict__ src, FieldGeom g, int nblocks,
                                                      float eps, uint32_t zkeep, uint8_t *__restrict__ slots,
                                                      uint32_t *__restrict__ nbytes,
                                                      const czw_field_t *__restrict__ fields, int nf,
@@ -138,32 +141,32 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
     __shared__ __align__(16) float A[LDS_WORDS];
     __shared__ uint32_t M[1024];
     __shared__ uint32_t cnt[2048];  // level-32 run counts, then their exclusive offsets
-    __shared__ uint32_t wsum[NT / 64];
+    __shared__ uint32_t wsum[CT / 64];
     __shared__ uint32_t s32_total;
     __shared__ int ticket[2];
     __shared__ czw_field_t T[LAYOUT == LAYOUT_BATCH ? BATCH_MAX_FIELDS : 1];
     const int t = threadIdx.x;
     const int G = gridDim.x;
     load_table<LAYOUT>(T, fields, nf);
-    float nx[NROW][32];
+    float nx[CR][32];
     int b = blockIdx.x;
     float eps_b = eps;
     if (b < nblocks) {
         const BlockRef<float> br = locate<LAYOUT, float>(T, nf, b, src, nullptr, g, eps);
         eps_b = br.eps;
 #pragma unroll
-        for (int k = 0; k < NROW; ++k) load_xrow_f32<RL>(br.src, br.g, br.lb, t + NT * k, nx[k]);
+        for (int k = 0; k < CR; ++k) load_xrow_f32<RL>(br.src, br.g, br.lb, t + CT * k, nx[k]);
     }
     for (int it = 0; b < nblocks; ++it) {
         uint8_t *slot = slots + (size_t)b * SLOT32;
         if (queue && t == 0) ticket[it & 1] = G + (int)atomicAdd(queue, 1u);
 #pragma unroll
-        for (int k = 0; k < NROW; ++k) {  // level 32, x-pass
+        for (int k = 0; k < CR; ++k) {  // level 32, x-pass
             float r[32];
 #pragma unroll
             for (int i = 0; i < 32; ++i) r[i] = nx[k][i];
             row<32, true, WT>(r);
-            lds_st<32>(A, xrow_lds_base<RL>(t + NT * k), 1, r);
+            lds_st<32>(A, xrow_lds_base<RL>(t + CT * k), 1, r);
         }
         if (t < 2048 - NRUN32) cnt[NRUN32 + t] = 0u;
         __syncthreads();
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
             load_xrow_f32<RL>(br.src, br.g, br.lb, t, nx[0]);
         }
         {  // level 32, y-pass: rows (z, x)
-            int base[NROW];
+            int base[CR];
             ypass_bases(t, base);
             pass_rows_f32<true, WT>(A, base, RP);
         }
@@ -190,13 +193,13 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
         // (y,x) all 32 results are final; inside it only z >= 16 is, and the
         // level-16 z-pass continues on z < 16.  Keep bits are taken here, in
         // registers; the values stay in registers until they are written.
-        float r[NROW][32];
-        uint32_t kb[NROW];
+        float r[CR][32];
+        uint32_t kb[CR];
 #pragma unroll
-        for (int k = 0; k < NROW; ++k) lds_ld<32>(A, ((t + NT * k) >> 5) * RP + ((t + NT * k) & 31), SP, r[k]);
+        for (int k = 0; k < CR; ++k) lds_ld<32>(A, ((t + CT * k) >> 5) * RP + ((t + CT * k) & 31), SP, r[k]);
 #pragma unroll
-        for (int k = 0; k < NROW; ++k) {
-            const int q = t + NT * k, y = q >> 5, x = q & 31;
+        for (int k = 0; k < CR; ++k) {
+            const int q = t + CT * k, y = q >> 5, x = q & 31;
             const bool corner = (y < 16) && (x < 16);
             row<32, true, WT>(r[k]);
             uint32_t bits = 0;
@@ -221,17 +224,18 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
         }
         __syncthreads();
         {  // exclusive scan of the run counts in stream order
-            uint32_t c4[4], sum = 0;
+            constexpr int PER = 2048 / CT;
+            uint32_t c4[PER], sum = 0;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                c4[i] = cnt[4 * t + i];
+            for (int i = 0; i < PER; ++i) {
+                c4[i] = cnt[PER * t + i];
                 sum += c4[i];
             }
             uint32_t S32;
-            uint32_t run = block_excl_scan(sum, wsum, &S32);
+            uint32_t run = block_excl_scan<CT>(sum, wsum, &S32);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                cnt[4 * t + i] = run;
+            for (int i = 0; i < PER; ++i) {
+                cnt[PER * t + i] = run;
                 run += c4[i];
             }
             if (t == 0) s32_total = S32;
@@ -249,8 +253,8 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
         uint32_t *st32 = staged ? reinterpret_cast<uint32_t *>(A) + STAGE32_AT : d32;
         const uint32_t dummy = (uint32_t)STAGE32 + (t & 63);
 #pragma unroll
-        for (int k = 0; k < NROW; ++k) {
-            const int q = t + NT * k, y = q >> 5, x = q & 31;
+        for (int k = 0; k < CR; ++k) {
+            const int q = t + CT * k, y = q >> 5, x = q & 31;
             const int c0 = 2 * (x >> 4) + 4 * (y >> 4);
             const int base = (y & 15) * 16 + (x & 15);
             uint32_t i0 = (c0 > 0) ? cnt[(c0 - 1) * 256 + base] : 0u;
@@ -275,15 +279,15 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
                 }
             }
         }
-        {  // second half of the prefetch
+        if constexpr (CR > 1) {  // second half of the prefetch
             const BlockRef<float> br = locate<LAYOUT, float>(T, nf, bn, src, nullptr, g, eps);
-            load_xrow_f32<RL>(br.src, br.g, br.lb, t + NT, nx[1]);
+            load_xrow_f32<RL>(br.src, br.g, br.lb, t + CT, nx[1]);
         }
         __syncthreads();  // stage complete, wsum free again
         if (staged) {
             const uint4 *s4 = reinterpret_cast<const uint4 *>(st32);
             uint4 *d4 = reinterpret_cast<uint4 *>(d32);
-            for (uint32_t i = t; i < S32 / 4; i += NT) {
+            for (uint32_t i = t; i < S32 / 4; i += CT) {
                 uint4 v = s4[i];
                 v.x &= zkeep;
                 v.y &= zkeep;
@@ -291,7 +295,7 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
                 v.w &= zkeep;
                 d4[i] = v;
             }
-            for (uint32_t i = (S32 & ~3u) + t; i < S32; i += NT) d32[i] = st32[i] & zkeep;
+            for (uint32_t i = (S32 & ~3u) + t; i < S32; i += CT) d32[i] = st32[i] & zkeep;
         }
         fwt16_8<WT, RP, SP>(A, t);
 
@@ -299,7 +303,7 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
         // thread (base 4^3 always kept); values re-read from A
         uint32_t keep = 0;
         float v[8];
-        corner_piece<RP, SP>(t, [&](int q, int h, int addr, int stride, int word, int shift) {
+        if (t < 512) corner_piece<RP, SP>(t, [&](int q, int h, int addr, int stride, int word, int shift) {
             uint32_t bits = 0;
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
@@ -313,7 +317,7 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
             atomicOr(&M[word], bits << shift);
         });
         uint32_t Sc;
-        uint32_t idx = block_excl_scan(__popc(keep), wsum, &Sc);
+        uint32_t idx = block_excl_scan<CT>(__popc(keep), wsum, &Sc);
         {
             uint32_t *dc = reinterpret_cast<uint32_t *>(slot + MASK_BYTES);
 #pragma unroll
@@ -322,7 +326,7 @@ __global__ __launch_bounds__(NT) void k_compress_f32(const float *__restrict__ s
         }
         __syncthreads();  // mask complete
 #pragma unroll
-        for (int k = 0; k < NROW; ++k) reinterpret_cast<uint32_t *>(slot)[t + NT * k] = M[t + NT * k];
+        for (int k = 0; k < CR; ++k) reinterpret_cast<uint32_t *>(slot)[t + CT * k] = M[t + CT * k];
         if (t == 0) {
             nbytes[b] = MASK_BYTES + 4u * (Sc + S32);
             nbytes[nblocks + b] = Sc;
@@ -428,7 +432,7 @@ static int compress_slots_f32(const float *src, int nblocks, FieldGeom g, float 
     const uint32_t zkeep = zerobits ? ~((1u << zerobits) - 1u) : 0xffffffffu;
     const void *fn = wt == 1 ? (const void *)k_compress_f32<1, LAYOUT>
                    : wt == 2 ? (const void *)k_compress_f32<2, LAYOUT> : (const void *)k_compress_f32<3, LAYOUT>;
-    const dim3 grid(std::min(nblocks, resident_groups(fn, NT))), blk(NT);
+    const dim3 grid(std::min(nblocks, resident_groups(fn, CT))), blk(CT);
     const czw_field_t *nf0 = nullptr;
     uint32_t *nq = nullptr;
     if (wt == 1) hipLaunchKernelGGL((k_compress_f32<1, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, nf0, 0, nq);
@@ -517,8 +521,8 @@ static int batch_compress(const czw_batch_t *B, hipStream_t s)
     const uint32_t zkeep = B->zerobits ? ~((1u << B->zerobits) - 1u) : 0xffffffffu;
     const void *fn = wt == 1 ? (const void *)k_compress_f32<1, LAYOUT_BATCH>
                    : wt == 2 ? (const void *)k_compress_f32<2, LAYOUT_BATCH> : (const void *)k_compress_f32<3, LAYOUT_BATCH>;
-    const int groups = B->groups ? B->groups : resident_groups(fn, NT);
-    const dim3 grid(std::min(B->nblocks, groups)), blk(NT);
+    const int groups = B->groups ? B->groups : resident_groups(fn, CT);
+    const dim3 grid(std::min(B->nblocks, groups)), blk(CT);
     const float *src0 = nullptr;
     const FieldGeom g{0, 0, 0};
     if (wt == 1) hipLaunchKernelGGL((k_compress_f32<1, LAYOUT_BATCH>), grid, blk, 0, s, src0, g, B->nblocks, 0.0f, zkeep, slots, nb, B->fields, B->nfields, q);
@@ -695,7 +699,7 @@ int czw_batch_autotune(czw_batch_t *batch, int stage, hipStream_t stream, float 
 {
     if (!batch_ok(batch) || (stage != 1 && stage != 2)) return CZW_EINVAL;
     const void *fn = (const void *)k_compress_f32<1, LAYOUT_BATCH>;
-    const int R = resident_groups(fn, NT);
+    const int R = resident_groups(fn, CT);
     const int shapes[4][2] = {{R, 0}, {R, 1}, {R / 2 > 0 ? R / 2 : 1, 0}, {R / 2 > 0 ? R / 2 : 1, 1}};
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return CZW_EHIP;
