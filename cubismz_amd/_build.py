@@ -55,7 +55,8 @@ def build_host():
     rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
     _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-Wall", "-D__HIP_PLATFORM_AMD__",
           "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(rocm, "include"),
-          os.path.join(CSRC, "cz_file.cpp"), os.path.join(CSRC, "cz_pipeline.cpp"), "-o", out + ".tmp",
+          os.path.join(CSRC, "cz_file.cpp"), os.path.join(CSRC, "cz_lz4.cpp"), os.path.join(CSRC, "cz_pipeline.cpp"),
+          "-o", out + ".tmp",
           "-L" + LIBDIR, "-lczw", "-lz", "-L" + os.path.join(rocm, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN"])
     os.replace(out + ".tmp", out)
     return out

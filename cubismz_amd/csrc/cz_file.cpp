@@ -38,6 +38,11 @@ int wavelet_type(const std::string &name)
     return -1;
 }
 
+size_t chunk_bound(int prec)
+{
+    return (size_t)entries_per_chunk(prec) * (4 + 4096 + (size_t)prec * 32768);
+}
+
 int entries_per_chunk(int prec)
 {
     // sizeof(WaveletCompressorGeneric<32,Real>) by layout arithmetic
@@ -74,7 +79,7 @@ std::string make_header(const Params &p)
     // the threshold member has type Real
     if (p.prec == 4) ss << "WaveletThreshold: " << (float)p.threshold << "\n";
     else ss << "WaveletThreshold: " << p.threshold << "\n";
-    ss << "Encoder: " << (p.encoder == ENC_ZLIB ? "zlib" : "none") << "\n";
+    ss << "Encoder: " << (p.encoder == ENC_ZLIB ? "zlib" : p.encoder == ENC_LZ4 ? "lz4" : "none") << "\n";
     ss << META_TITLE;
     return ss.str();
 }
@@ -129,7 +134,7 @@ bool inflate_chunk(const uint8_t *in, size_t n, std::vector<uint8_t> &out)
 int encode_chunks(const Params &p, const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads,
                   std::vector<std::vector<uint8_t>> &chunks)
 {
-    if (nblocks <= 0) return -1;
+    if (nblocks <= 0 || (p.encoder != ENC_NONE && p.encoder != ENC_ZLIB && p.encoder != ENC_LZ4)) return -1;
     const int E = entries_per_chunk(p.prec);
     const int nchunks = (nblocks + E - 1) / E;
     chunks.assign(nchunks, std::vector<uint8_t>());
@@ -143,6 +148,8 @@ int encode_chunks(const Params &p, const uint8_t *records, const uint64_t *offse
         const uint64_t b = offsets[std::min<size_t>((size_t)(c + 1) * E, (size_t)nblocks)];
         if (p.encoder == ENC_ZLIB) {
             if (!deflate_chunk(records + a, b - a, chunks[c])) bad = 1;
+        } else if (p.encoder == ENC_LZ4) {
+            lz4_compress(records + a, b - a, chunks[c]);
         } else {
             chunks[c].assign(records + a, records + b);
         }
@@ -262,8 +269,9 @@ int parse_cz(const std::vector<uint8_t> &file, FileInfo &info, std::string *msg)
     p.wtype = (wav == "none") ? -1 : wavelet_type(wav);
     if (wav != "none" && p.wtype < 0) return fail("unknown wavelet type");
     if (enc == "zlib") p.encoder = ENC_ZLIB;
+    else if (enc == "lz4") p.encoder = ENC_LZ4;
     else if (enc == "none") p.encoder = ENC_NONE;
-    else return fail("unsupported encoder (only zlib and none)");
+    else return fail("unsupported encoder (zlib, lz4 and none are known)");
 
     const long long nb = (long long)p.bpd[0] * p.bpd[1] * p.bpd[2];
     const long long bps = (long long)p.sub_bpd[0] * p.sub_bpd[1] * p.sub_bpd[2];
@@ -333,6 +341,8 @@ int gather_records(const std::vector<uint8_t> &file, const FileInfo &info, int n
         const uint8_t *src = file.data() + info.chunk_start[c];
         if (info.p.encoder == ENC_ZLIB) {
             if (!inflate_chunk(src, info.chunk_size[c], raw[c])) bad = 1;
+        } else if (info.p.encoder == ENC_LZ4) {
+            if (!lz4_decompress(src, info.chunk_size[c], raw[c], chunk_bound(info.p.prec))) bad = 1;
         } else {
             raw[c].assign(src, src + info.chunk_size[c]);
         }
@@ -491,6 +501,27 @@ int czh_gather(czh_file *h, int nthreads, uint8_t **records, uint64_t *records_b
 }
 
 void czh_free(void *p) { std::free(p); }
+
+int czh_lz4_compress(const uint8_t *in, uint64_t n, uint8_t *out, uint64_t cap, uint64_t *produced)
+{
+    if ((!in && n) || !out || !produced || n > 0x7fffffffull) return CZH_EINVAL;
+    std::vector<uint8_t> o;
+    cz::lz4_compress(in, (size_t)n, o);
+    if (o.size() > cap) return CZH_EINVAL;
+    std::memcpy(out, o.data(), o.size());
+    *produced = o.size();
+    return CZH_OK;
+}
+
+int czh_lz4_decompress(const uint8_t *in, uint64_t n, uint8_t *out, uint64_t cap, uint64_t *produced)
+{
+    if ((!in && n) || !produced) return CZH_EINVAL;
+    std::vector<uint8_t> o;
+    if (!cz::lz4_decompress(in, (size_t)n, o, (size_t)cap)) return CZH_ECORRUPT;
+    if (!o.empty()) std::memcpy(out, o.data(), o.size());
+    *produced = o.size();
+    return CZH_OK;
+}
 
 int czh_deflate(const uint8_t *in, uint64_t n, uint8_t *out, uint64_t cap, uint64_t *produced)
 {

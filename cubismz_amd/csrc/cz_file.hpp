@@ -44,7 +44,7 @@ static_assert(sizeof(CompressedBlock) == 20, "CompressedBlock must be 20 bytes")
 extern const char *const OCEAN_TITLE;  // 48 bytes
 extern const char *const LUT_TITLE;    // 46 bytes
 
-enum Encoder { ENC_NONE = 0, ENC_ZLIB = 1 };
+enum Encoder { ENC_NONE = 0, ENC_ZLIB = 1, ENC_LZ4 = 2 };
 
 // Name recorded for a wavelet type (FullWaveletTransform.h:28-35); wtype < 0
 // means no wavelet stage (raw blocks, the reference's "default" build).
@@ -72,6 +72,13 @@ std::string make_header(const Params &p);
 bool deflate_chunk(const uint8_t *in, size_t n, std::vector<uint8_t> &out);
 // Inflate a whole chunk; returns false unless inflate ends with Z_STREAM_END.
 bool inflate_chunk(const uint8_t *in, size_t n, std::vector<uint8_t> &out);
+// The LZ4 stage (CompressionEncoders.h:192-238 with the vendored LZ4 of
+// ThirdParty/lz4): the same greedy parse as its LZ4_compress, and a block
+// decoder bounded by max_out (cz_lz4.cpp).
+void lz4_compress(const uint8_t *src, size_t n, std::vector<uint8_t> &dst);
+bool lz4_decompress(const uint8_t *src, size_t n, std::vector<uint8_t> &dst, size_t max_out);
+// Bytes of one writer chunk's records at most (ENTRIES records of a full block).
+size_t chunk_bound(int prec);
 
 // Serialize one subdomain's record stream (block order, x fastest) into a
 // complete single-subdomain .cz image.  records/offsets as in czw.h.

@@ -63,7 +63,8 @@ extern "C" int czh_compress_device_field(const char *path, const void *field, in
                                          int slab_blocks, int overlap, hipStream_t stream, uint64_t *file_bytes,
                                          double *times)
 {
-    if (!field || (prec != 4 && prec != 8) || nx <= 0 || ny <= 0 || nz <= 0 || (nx | ny | nz) & 31)
+    if (!field || (prec != 4 && prec != 8) || nx <= 0 || ny <= 0 || nz <= 0 || (nx | ny | nz) & 31 ||
+        (encoder != CZH_ENCODER_NONE && encoder != CZH_ENCODER_ZLIB && encoder != CZH_ENCODER_LZ4))
         return CZH_EINVAL;
     const auto t0 = std::chrono::steady_clock::now();
     const int bpd[3] = {nx / 32, ny / 32, nz / 32};

@@ -190,7 +190,7 @@ bool decode_cz(const std::string &path, int expect_wtype, Field &f, bool verbose
     if (verbose)
         std::printf("%s: Blocks: %d x %d x %d, sizeofReal %d, Wavelets: %s, WaveletThreshold: %g, Encoder: %s\n",
                     path.c_str(), p.bpd[0], p.bpd[1], p.bpd[2], p.prec, cz::wavelet_name(p.wtype), p.threshold,
-                    p.encoder == cz::ENC_ZLIB ? "zlib" : "none");
+                    p.encoder == cz::ENC_ZLIB ? "zlib" : p.encoder == cz::ENC_LZ4 ? "lz4" : "none");
     std::vector<uint8_t> rec;
     std::vector<uint64_t> off;
     if (cz::gather_records(bytes, info, 0, rec, off, &msg)) {
@@ -253,7 +253,7 @@ bool decode_cz(const std::string &path, int expect_wtype, Field &f, bool verbose
 
 #if CZ_TOOL == 1
 // hdf2cz (Tools/hdf2cz.cpp, Test_IO_Compressed.h:249-342).  Extra flags:
-// -precision float|double (raw input), -zerobits, -shuffle, -encoder zlib|none,
+// -precision float|double (raw input), -zerobits, -shuffle, -encoder zlib|lz4|none,
 // -nowavelets (raw blocks, like the reference's default build).
 int tool_main(int argc, char **argv)
 {
@@ -262,7 +262,7 @@ int tool_main(int argc, char **argv)
     if (a.has("-help") || out == "none" || (!a.has("-h5file") && !a.has("-rawfile"))) {
         std::printf("Usage: %s -h5file <hdf5 file> | -rawfile <raw file> -czfile <cz file> -threshold <e> [-wtype <wt>] "
                     "[-bpdx <nbx>] [-bpdy <nby>] [-bpdz <nbz>] [-precision float|double] [-zerobits <z>] "
-                    "[-shuffle] [-encoder zlib|none] [-nowavelets]\n", "hdf2cz");
+                    "[-shuffle] [-encoder zlib|lz4|none] [-nowavelets]\n", "hdf2cz");
         return 1;
     }
     if (a.i("-nprocx", 1) * a.i("-nprocy", 1) * a.i("-nprocz", 1) != 1) {
@@ -287,7 +287,14 @@ int tool_main(int argc, char **argv)
     for (int d = 0; d < 3; ++d) p.bpd[d] = p.sub_bpd[d] = f.n[d] / 32;
     p.wtype = wtype;
     p.threshold = threshold;
-    p.encoder = a.str("-encoder", "zlib") == "none" ? cz::ENC_NONE : cz::ENC_ZLIB;
+    {
+        const std::string e = a.str("-encoder", "zlib");
+        if (e != "zlib" && e != "lz4" && e != "none") {
+            std::fprintf(stderr, "hdf2cz: unknown -encoder %s (zlib, lz4 or none)\n", e.c_str());
+            return 1;
+        }
+        p.encoder = e == "none" ? cz::ENC_NONE : e == "lz4" ? cz::ENC_LZ4 : cz::ENC_ZLIB;
+    }
     const int nb = p.bpd[0] * p.bpd[1] * p.bpd[2];
     size_t file_size = 0;
     if (wtype < 0) {  // raw blocks [z][y][x]

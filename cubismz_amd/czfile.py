@@ -14,7 +14,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "lib", "libczhost.so")
-ENCODERS = {"none": 0, "zlib": 1}
+ENCODERS = {"none": 0, "zlib": 1, "lz4": 2}
 
 _lib = None
 
@@ -41,6 +41,8 @@ def load_library():
         L.czh_gather.argtypes = [vp, i, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64), C.POINTER(C.c_void_p)]
         L.czh_free.argtypes = [vp]
         L.czh_deflate.argtypes = [vp, C.c_uint64, vp, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.czh_lz4_compress.argtypes = [vp, C.c_uint64, vp, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.czh_lz4_decompress.argtypes = [vp, C.c_uint64, vp, C.c_uint64, C.POINTER(C.c_uint64)]
         L.czh_compress_device_field.argtypes = [C.c_char_p, vp, i, i, i, i, C.c_float, i, i, i, i, i, i, i, vp,
                                                 C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
         _lib = L
@@ -89,6 +91,27 @@ def compress_device_field(path, field, eps, wtype=3, zerobits=0, shuffle=False, 
         raise CzFileError(f"czh_compress_device_field failed ({rc})")
     keys = ("wall", "device", "zlib", "copy", "assemble")
     return int(nbytes.value), {k: float(v) for k, v in zip(keys, times)}
+
+
+def lz4_compress(data):
+    """The writer's per-chunk LZ4 stage."""
+    d = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8))
+    out = np.empty(d.size + d.size // 255 + 16, np.uint8)
+    n = C.c_uint64()
+    if load_library().czh_lz4_compress(d.ctypes.data if d.size else None, d.size, out.ctypes.data, out.size,
+                                       C.byref(n)):
+        raise CzFileError("lz4 compress failed")
+    return out[:n.value].tobytes()
+
+
+def lz4_decompress(data, max_out):
+    d = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8))
+    out = np.empty(max(1, int(max_out)), np.uint8)
+    n = C.c_uint64()
+    if load_library().czh_lz4_decompress(d.ctypes.data if d.size else None, d.size, out.ctypes.data, int(max_out),
+                                         C.byref(n)):
+        raise CzFileError("corrupt lz4 block")
+    return out[:n.value].tobytes()
 
 
 def deflate(data):

@@ -25,7 +25,7 @@ extern "C" {
 #endif
 
 enum { CZH_OK = 0, CZH_EINVAL = -1, CZH_EIO = -2, CZH_EZLIB = -3, CZH_ECORRUPT = -4 };
-enum { CZH_ENCODER_NONE = 0, CZH_ENCODER_ZLIB = 1 };
+enum { CZH_ENCODER_NONE = 0, CZH_ENCODER_ZLIB = 1, CZH_ENCODER_LZ4 = 2 };
 
 typedef struct czh_file czh_file;
 
@@ -54,6 +54,13 @@ void czh_free(void *p);
 
 /* The writer's per-chunk deflate (zlib level Z_DEFAULT_COMPRESSION, Z_FINISH). */
 int czh_deflate(const uint8_t *in, uint64_t n, uint8_t *out, uint64_t cap, uint64_t *produced);
+
+/* The writer's LZ4 stage on one chunk: the greedy parse of the LZ4_compress
+ * the reference vendors (ThirdParty/lz4/lz4.c, called at
+ * CompressionEncoders.h:230-231), and the block decoder its reader uses
+ * (LZ4_uncompress_unknownOutputSize, CompressionEncoders.h:75-76). */
+int czh_lz4_compress(const uint8_t *in, uint64_t n, uint8_t *out, uint64_t cap, uint64_t *produced);
+int czh_lz4_decompress(const uint8_t *in, uint64_t n, uint8_t *out, uint64_t cap, uint64_t *produced);
 
 /* Compress an HDF5-ordered DEVICE field [nx][ny][nz] of Real (prec bytes)
  * straight to a single-writer .cz file (path may be NULL: only the size is

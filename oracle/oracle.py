@@ -244,3 +244,22 @@ def ref_row(x, wtype, forward=True):
     a = np.ascontiguousarray(x, dtype=dt).copy()
     ref(s).rf_row(_ptr(a), a.size, wtype, int(bool(forward)))
     return a
+
+
+def ref_lz4_compress(data):
+    """LZ4_compress of the LZ4 release vendored by the reference (oracle/_ref)."""
+    path = os.path.join(HERE, "_ref", "libczref_lz4.so")
+    L = _refs.get(path)
+    if L is None:
+        L = C.CDLL(path)
+        L.LZ4_compress.restype = C.c_int
+        L.LZ4_compress.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+        _refs[path] = L
+    src = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8))
+    out = np.empty(src.size + src.size // 255 + 16, np.uint8)
+    n = L.LZ4_compress(_ptr(src), _ptr(out), int(src.size))
+    return out[:n].tobytes()
+
+
+def ref_lz4_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libczref_lz4.so"))

@@ -16,6 +16,12 @@ compiled from its sources by oracle/Makefile (oracle/_ref/libczref_*.so).
       payload size and sha256 (compress) and sha256 of the decoded block.
   payloads.npz
       a few full reference payloads, for byte-level diffs when a hash fails.
+  lz4.json / lz4.npz
+      LZ4_compress (ThirdParty/lz4/lz4.c, built as oracle/_ref/libczref_lz4.so)
+      of generated byte strings (lz4_generated_inputs) and of two writer chunks
+      of demo records ([int32 nbytes][payload] x 15 f32 / 7 f64, stored in
+      lz4.npz as inputs): output size and sha256, and the output itself when
+      small.
 
 Usage: python tests/golden/make_golden.py [/path/to/reference]
 """
@@ -83,6 +89,42 @@ EPS = [1e-2, 1e-3, 1e-4, 5e-5]
 VARIANTS = [(0, False), (4, False), (0, True), (4, True)]
 
 
+def lz4_generated_inputs():
+    """Byte strings around the LZ4 parse's edges: empty, below/at the 13-byte
+    minimum, either side of the 65547-byte switch between the 16-bit and the
+    32-bit hash table, long zero runs (extended match lengths), long literal
+    runs (extended literal lengths) and low-entropy noise (many short matches)."""
+    def noise(seed, n, levels):
+        return (splitmix_uniform(seed, n) * levels).astype(np.uint8).tobytes()
+    text = b"CubismZ wavelet records, chunk after chunk. " * 1500
+    return {
+        "empty": b"",
+        "n12": noise(1, 12, 256),
+        "n13": noise(2, 13, 2),
+        "text_1000": text[:1000],
+        "noise256_5000": noise(3, 5000, 256),
+        "noise4_65546": noise(4, 65546, 4),
+        "noise4_65547": noise(5, 65547, 4),
+        "noise16_200000": noise(6, 200000, 16),
+        "zeros_300000": bytes(300000),
+        "text_66000": text[:66000],
+        "zeros_then_noise": bytes(70000) + noise(7, 70000, 256) + bytes(1000),
+    }
+
+
+def records_chunk(demo, prec, nrec, eps, compress):
+    """One writer chunk: nrec records [int32 nbytes][payload] of consecutive demo blocks."""
+    dt = np.float32 if prec == "f32" else np.float64
+    parts = []
+    for b in range(nrec):
+        ix, iy, iz = b % 4, (b // 4) % 4, b // 16
+        blk = np.ascontiguousarray(demo[32 * ix:32 * ix + 32, 32 * iy:32 * iy + 32, 32 * iz:32 * iz + 32]
+                                   .transpose(2, 1, 0)).astype(dt)
+        pay = np.asarray(compress(blk, eps, 1, 0, False), np.uint8)
+        parts.append(np.int32(pay.size).tobytes() + pay.tobytes())
+    return b"".join(parts)
+
+
 def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
@@ -139,6 +181,22 @@ def main():
     with open(os.path.join(HERE, "blocks.json"), "w") as f:
         json.dump(table, f, indent=0)
     np.savez_compressed(os.path.join(HERE, "payloads.npz"), **keep)
+
+    if not O.ref_lz4_available():
+        raise SystemExit("oracle/_ref/libczref_lz4.so is not built: run make -C oracle")
+    lz_in = lz4_generated_inputs()
+    chunks = {"records_f32_15": records_chunk(demo32, "f32", 15, 1e-3, O.ref_compress),
+              "records_f64_7": records_chunk(demo64, "f64", 7, 1e-6, O.ref_compress)}
+    lz_in.update(chunks)
+    lz_table, lz_keep = {}, {f"in_{k}": np.frombuffer(v, np.uint8) for k, v in chunks.items()}
+    for name, data in lz_in.items():
+        out = O.ref_lz4_compress(data)
+        lz_table[name] = {"n_in": len(data), "n_out": len(out), "sha256": hashlib.sha256(out).hexdigest()}
+        if len(out) <= 4096:
+            lz_keep[f"out_{name}"] = np.frombuffer(out, np.uint8)
+    with open(os.path.join(HERE, "lz4.json"), "w") as f:
+        json.dump(lz_table, f, indent=1)
+    np.savez_compressed(os.path.join(HERE, "lz4.npz"), **lz_keep)
     print("wrote", sorted(os.listdir(HERE)))
 
 

@@ -193,3 +193,102 @@ def test_multi_subdomain_reader(oracle, host, tmp_path):
         o = int(off[g])
         n = struct.unpack_from("<i", rec.tobytes(), o)[0]
         assert rec[o + 4:o + 4 + n].tobytes() == pays[g].tobytes()
+
+
+def _lz4_cases():
+    from tests.golden.make_golden import lz4_generated_inputs
+    inputs = lz4_generated_inputs()
+    z = np.load(os.path.join(GOLD, "lz4.npz"))
+    for k in z.files:
+        if k.startswith("in_"):
+            inputs[k[3:]] = z[k].tobytes()
+    return inputs, {k[4:]: z[k].tobytes() for k in z.files if k.startswith("out_")}
+
+
+def test_lz4_matches_reference_golden(host):
+    """The writer's LZ4 stage gives the bytes of the vendored LZ4_compress
+    (ThirdParty/lz4/lz4.c:695, called at CompressionEncoders.h:230) on the
+    golden inputs: parse edges, both hash tables, and real record chunks."""
+    import hashlib
+    import json
+    table = json.load(open(os.path.join(GOLD, "lz4.json")))
+    inputs, outs = _lz4_cases()
+    assert set(inputs) == set(table)
+    for name, data in inputs.items():
+        want = table[name]
+        assert len(data) == want["n_in"], name
+        got = czfile.lz4_compress(data)
+        if name in outs:
+            assert got == outs[name], name
+        assert (len(got), hashlib.sha256(got).hexdigest()) == (want["n_out"], want["sha256"]), name
+        assert czfile.lz4_decompress(got, len(data)) == data, name
+
+
+def test_lz4_matches_built_reference(host, oracle):
+    """Against the reference LZ4 itself (oracle/_ref), where it is built, on
+    random inputs around the table switch and on sparse and repetitive data."""
+    if not oracle.ref_lz4_available():
+        pytest.skip("oracle/_ref not built here")
+    rng = np.random.default_rng(4)
+    for n in (13, 14, 255, 270, 4096, 65535, 65546, 65547, 65548, 131072, 1 << 20):
+        for levels in (2, 7, 256):
+            data = rng.integers(0, levels, n, dtype=np.uint8)
+            if levels == 7:
+                data[rng.random(n) < 0.9] = 0
+            d = data.tobytes()
+            assert czfile.lz4_compress(d) == oracle.ref_lz4_compress(d), (n, levels)
+
+
+def test_lz4_decoder_rejects_corruption(host):
+    data = (b"abcd" * 5000) + bytes(range(256)) * 4
+    good = czfile.lz4_compress(data)
+    assert czfile.lz4_decompress(good, len(data)) == data
+    bad_cases = [
+        good[:-1],                      # truncated literals
+        good[:3],                       # truncated in a sequence
+        good + b"\x00",                 # trailing token with no end
+        b"\x0f\x00\x00",                # match before any output
+    ]
+    for b in bad_cases:
+        with pytest.raises(czfile.CzFileError):
+            czfile.lz4_decompress(b, len(data))
+    with pytest.raises(czfile.CzFileError):
+        czfile.lz4_decompress(good, len(data) - 1)  # output bound too small
+
+
+def test_lz4_container_roundtrip(oracle, host, tmp_path):
+    """A .cz written with the LZ4 stage ("Encoder: lz4", SerializerIO...:548-549)
+    reads back record for record, and its chunks are the LZ4 of the zlib
+    file's inflated chunks."""
+    d = demo("f64")[:64, :64, :64]
+    slots, nb = oracle.compress_field(np.ascontiguousarray(d), 1e-6, 1)
+    rec, off = czfile.records_from_slots(slots, nb)
+    p4 = tmp_path / "a_lz4.cz"
+    pz = tmp_path / "a_zlib.cz"
+    czfile.write(str(p4), rec, off, (2, 2, 2), 8, 1, 1e-6, encoder="lz4")
+    czfile.write(str(pz), rec, off, (2, 2, 2), 8, 1, 1e-6, encoder="zlib")
+    assert b"Encoder: lz4\n" in p4.read_bytes()
+    with czfile.CzFile(str(p4)) as f:
+        assert f.encoder == "lz4"
+        r2, o2 = f.records()
+    assert np.array_equal(r2, rec) and np.array_equal(o2, off)
+    # the ocean holds LZ4_compress of each chunk of whole records
+    e = czfile.entries_per_chunk(8)
+    b = p4.read_bytes()
+    start = b.index(b"START-BINARY-OCEAN==============\n") + len(b"START-BINARY-OCEAN==============\n")
+    first = rec[int(off[0]):int(off[min(e, len(off) - 1)])].tobytes()
+    c0 = czfile.lz4_compress(first)
+    assert b[start:start + len(c0)] == c0
+
+
+def test_writer_rejects_unknown_encoder(host, tmp_path):
+    rec, off = czfile.records_from_slots(np.zeros((1, 32768 * 4), np.uint8), np.array([32768 * 4]))
+    old = dict(czfile.ENCODERS)
+    try:
+        czfile.ENCODERS["bogus"] = 7
+        with pytest.raises(czfile.CzFileError):
+            czfile.write(str(tmp_path / "x.cz"), rec, off, (1, 1, 1), 4, -1, 0.0, encoder="bogus")
+    finally:
+        czfile.ENCODERS.clear()
+        czfile.ENCODERS.update(old)
+    assert not (tmp_path / "x.cz").exists()

@@ -45,3 +45,27 @@ def test_device_writer_equals_whole_field_write(torch_gpu, tmp_path, dt, slab, o
     assert nbytes == got.stat().st_size
     assert got.read_bytes() == ref.read_bytes()
     assert times["wall"] > 0 and times["device"] > 0
+
+
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+@pytest.mark.parametrize("encoder", ["lz4", "none"])
+def test_device_writer_other_encoders(torch_gpu, tmp_path, dt, encoder):
+    """The overlapped writer with the LZ4 and the uncompressed second stage
+    writes the bytes of the whole-field writer with that stage."""
+    torch = torch_gpu
+    dtype = torch.float32 if dt == "f32" else torch.float64
+    prec = 4 if dt == "f32" else 8
+    print(f"[case] device writer {dt} encoder={encoder}", flush=True)
+    f = fields.bubble_field(SHAPE, seed=9, nbubbles=5, dtype=dtype, device="cuda")
+    rec, off = czw.compress_field(f, 1e-4, 1, 0, False)
+    o = off.cpu().numpy()
+    r = rec[: int(o[-1])].cpu().numpy()
+    ref = tmp_path / "ref.cz"
+    czfile.write(str(ref), r, o, (3, 2, 4), prec, 1, 1e-4, encoder=encoder)
+    got = tmp_path / "got.cz"
+    czfile.compress_device_field(str(got), f, 1e-4, 1, encoder=encoder, slab_blocks=2 * czfile.entries_per_chunk(prec))
+    assert got.read_bytes() == ref.read_bytes()
+    with czfile.CzFile(str(got)) as cf:
+        assert cf.encoder == encoder
+        r2, o2 = cf.records()
+    assert np.array_equal(r2, r) and np.array_equal(o2, o)

@@ -79,3 +79,16 @@ def test_tools_other_wavelets_match_oracle(oracle, demo_raw, tmp_path, wtype):
         o = int(off[b])
         n = int(rec[o:o + 4].view(np.int32)[0])
         assert n == nb[b] and np.array_equal(rec[o + 4:o + 4 + n], slots[b, :n])
+
+
+def test_lz4_encoder_tools_roundtrip(demo_raw, tmp_path):
+    """hdf2cz -encoder lz4 (the reference's _USE_LZ4_ build) and cz2diff on
+    its file: the error columns of the published line are unchanged, only
+    the file size and ratio depend on the second stage."""
+    _tool("hdf2cz", "-rawfile", str(demo_raw), "-bpdx", "4", "-bpdy", "4", "-bpdz", "4", "-czfile", "l.cz",
+          "-fpath", str(tmp_path), "-threshold", "0.00005", "-wtype", "3", "-encoder", "lz4")
+    with czfile.CzFile(str(tmp_path / "l.cz")) as f:
+        assert f.encoder == "lz4"
+    out = _tool("cz2diff", "-czfile1", str(tmp_path / "l.cz"), "-wtype", "3", "-rawfile2", str(demo_raw))
+    res = [l for l in out.splitlines() if l.startswith("RES:")][-1].split()
+    assert res[2:7] == PUBLISHED_F32.split()[2:7], out
