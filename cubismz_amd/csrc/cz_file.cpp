@@ -131,12 +131,39 @@ bool inflate_chunk(const uint8_t *in, size_t n, std::vector<uint8_t> &out)
     return true;
 }
 
+bool subdomains_valid(const Params &p)
+{
+    for (int d = 0; d < 3; ++d)
+        if (p.bpd[d] <= 0 || p.sub_bpd[d] <= 0 || p.bpd[d] % p.sub_bpd[d]) return false;
+    return true;
+}
+
+void file_block_order(const Params &p, std::vector<int> &order)
+{
+    const int sx = p.sub_bpd[0], sy = p.sub_bpd[1], sz = p.sub_bpd[2];
+    const int py = p.bpd[1] / sy, pz = p.bpd[2] / sz;
+    const int bps = sx * sy * sz, nb = p.bpd[0] * p.bpd[1] * p.bpd[2];
+    order.resize(nb);
+    for (int q = 0; q < nb; ++q) {
+        // rank q / bps at Cartesian coordinates (cx, cy, cz), the last one
+        // fastest (MPI_Cart_create, GridMPI.h:73-75); its blocks x fastest
+        // (Grid.h:80-85), offset by coordinates * subdomain blocks (GridMPI.h:111-114)
+        const int r = q / bps, i = q % bps;
+        const int cz = r % pz, cy = (r / pz) % py, cx = r / (py * pz);
+        const int gx = cx * sx + i % sx, gy = cy * sy + (i / sx) % sy, gz = cz * sz + i / (sx * sy);
+        order[q] = gx + p.bpd[0] * (gy + p.bpd[1] * gz);
+    }
+}
+
 int encode_chunks(const Params &p, const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads,
-                  std::vector<std::vector<uint8_t>> &chunks)
+                  std::vector<std::vector<uint8_t>> &chunks, const int *order, int per_subdomain)
 {
     if (nblocks <= 0 || (p.encoder != ENC_NONE && p.encoder != ENC_ZLIB && p.encoder != ENC_LZ4)) return -1;
     const int E = entries_per_chunk(p.prec);
-    const int nchunks = (nblocks + E - 1) / E;
+    const int bps = per_subdomain > 0 ? per_subdomain : nblocks;
+    if (nblocks % bps) return -1;
+    const int ncs = (bps + E - 1) / E;  // chunks per subdomain
+    const int nchunks = ncs * (nblocks / bps);
     chunks.assign(nchunks, std::vector<uint8_t>());
     int bad = 0;
 #ifdef _OPENMP
@@ -144,14 +171,27 @@ int encode_chunks(const Params &p, const uint8_t *records, const uint64_t *offse
 #pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
 #endif
     for (int c = 0; c < nchunks; ++c) {
-        const uint64_t a = offsets[(size_t)c * E];
-        const uint64_t b = offsets[std::min<size_t>((size_t)(c + 1) * E, (size_t)nblocks)];
+        // file positions [q0, q1) of this chunk's blocks
+        const int q0 = (c / ncs) * bps + (c % ncs) * E;
+        const int q1 = (c / ncs) * bps + std::min((c % ncs + 1) * E, bps);
+        const uint8_t *src;
+        size_t len;
+        std::vector<uint8_t> gathered;
+        if (!order) {
+            src = records + offsets[q0];
+            len = offsets[q1] - offsets[q0];
+        } else {  // the records of one chunk are not adjacent in block order
+            for (int q = q0; q < q1; ++q)
+                gathered.insert(gathered.end(), records + offsets[order[q]], records + offsets[order[q] + 1]);
+            src = gathered.data();
+            len = gathered.size();
+        }
         if (p.encoder == ENC_ZLIB) {
-            if (!deflate_chunk(records + a, b - a, chunks[c])) bad = 1;
+            if (!deflate_chunk(src, len, chunks[c])) bad = 1;
         } else if (p.encoder == ENC_LZ4) {
-            lz4_compress(records + a, b - a, chunks[c]);
+            lz4_compress(src, len, chunks[c]);
         } else {
-            chunks[c].assign(records + a, records + b);
+            chunks[c].assign(src, src + len);
         }
     }
     (void)nthreads;
@@ -162,49 +202,74 @@ int assemble_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks
                 std::vector<uint8_t> &file)
 {
     const int E = entries_per_chunk(p.prec);
-    const int nchunks = (int)chunks.size();
-    if (nblocks <= 0 || p.bpd[0] * p.bpd[1] * p.bpd[2] != nblocks || nchunks != (nblocks + E - 1) / E) return -1;
+    if (!subdomains_valid(p) || nblocks <= 0 || p.bpd[0] * p.bpd[1] * p.bpd[2] != nblocks) return -1;
+    const int bps = p.sub_bpd[0] * p.sub_bpd[1] * p.sub_bpd[2];
+    const int nsub = nblocks / bps, ncs = (bps + E - 1) / E;
+    if ((int)chunks.size() != nsub * ncs) return -1;
     const std::string ocean(OCEAN_TITLE), luttitle(LUT_TITLE);
     const std::string header = make_header(p);
-    size_t written = 0;
-    std::vector<uint64_t> lut(nchunks);
-    for (int c = 0; c < nchunks; ++c) {
-        lut[c] = written;
-        written += chunks[c].size();
+
+    // each subdomain writes its chunks, then uint64 lut[ncs] of chunk starts
+    // relative to its first byte; HeaderLUT {bytes incl. the lut, ncs}
+    // (SerializerIO_..._MPI_Simple.h:563-575); subdomains follow each other
+    // in rank order (MPI_Exscan of the byte counts, :385-401)
+    std::vector<HeaderLUT> hls(nsub);
+    size_t ocean_bytes = 0;
+    for (int s = 0; s < nsub; ++s) {
+        size_t w = 0;
+        for (int c = 0; c < ncs; ++c) w += chunks[(size_t)s * ncs + c].size();
+        hls[s] = HeaderLUT{(uint64_t)(w + sizeof(uint64_t) * ncs), ncs};
+        ocean_bytes += hls[s].aggregate_bytes;
     }
-    const size_t aggregate = written + sizeof(uint64_t) * nchunks;
-    const uint64_t header_disp = sizeof(uint64_t) + ocean.size() + aggregate;
+    const uint64_t header_disp = sizeof(uint64_t) + ocean.size() + ocean_bytes;
 
     file.clear();
-    file.reserve(header_disp + header.size() + nblocks * sizeof(BlockMetadata) + luttitle.size() + sizeof(HeaderLUT));
+    file.reserve(header_disp + header.size() + nblocks * sizeof(BlockMetadata) + luttitle.size() +
+                 nsub * sizeof(HeaderLUT));
     auto put = [&](const void *d, size_t n) {
         const uint8_t *q = static_cast<const uint8_t *>(d);
         file.insert(file.end(), q, q + n);
     };
     put(&header_disp, sizeof(header_disp));
     put(ocean.data(), ocean.size());
-    for (auto &c : chunks) put(c.data(), c.size());
-    put(lut.data(), sizeof(uint64_t) * nchunks);
+    std::vector<uint64_t> lut(ncs);
+    for (int s = 0; s < nsub; ++s) {
+        uint64_t w = 0;
+        for (int c = 0; c < ncs; ++c) {
+            const auto &ch = chunks[(size_t)s * ncs + c];
+            lut[c] = w;
+            w += ch.size();
+            put(ch.data(), ch.size());
+        }
+        put(lut.data(), sizeof(uint64_t) * ncs);
+    }
     put(header.data(), header.size());
-    // BlockMetadata {chunk, index in chunk, ix, iy, iz} in block order
-    // (SerializerIO_..._MPI_Simple.h:169-177, 328-332), blocks x fastest
+    // per subdomain, BlockMetadata {chunk, index in chunk, global ix, iy, iz}
+    // of its blocks in its own order (SerializerIO_..._MPI_Simple.h:169-177,
+    // 328-332; written at rank * NBLOCKS, :431-441)
+    std::vector<int> order;
+    file_block_order(p, order);
     const int bx = p.bpd[0], by = p.bpd[1];
-    for (int i = 0; i < nblocks; ++i) {
-        const BlockMetadata m = {i / E, i % E, i % bx, (i / bx) % by, i / (bx * by)};
+    for (int q = 0; q < nblocks; ++q) {
+        const int i = q % bps, g = order[q];
+        const BlockMetadata m = {i / E, i % E, g % bx, (g / bx) % by, g / (bx * by)};
         put(&m, sizeof(m));
     }
     put(luttitle.data(), luttitle.size());
-    const HeaderLUT hl = {(uint64_t)aggregate, nchunks};
-    put(&hl, sizeof(hl));
+    put(hls.data(), sizeof(HeaderLUT) * nsub);
     return 0;
 }
 
 int build_cz(const Params &p, const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads,
              std::vector<uint8_t> &file)
 {
-    if (nblocks <= 0 || p.bpd[0] * p.bpd[1] * p.bpd[2] != nblocks) return -1;
+    if (!subdomains_valid(p) || nblocks <= 0 || p.bpd[0] * p.bpd[1] * p.bpd[2] != nblocks) return -1;
+    const int bps = p.sub_bpd[0] * p.sub_bpd[1] * p.sub_bpd[2];
+    std::vector<int> order;
+    if (bps != nblocks) file_block_order(p, order);
     std::vector<std::vector<uint8_t>> chunks;
-    const int rc = encode_chunks(p, records, offsets, nblocks, nthreads, chunks);
+    const int rc = encode_chunks(p, records, offsets, nblocks, nthreads, chunks, order.empty() ? nullptr : order.data(),
+                                 bps);
     if (rc) return rc;
     return assemble_cz(p, chunks, nblocks, file);
 }
@@ -436,10 +501,20 @@ int czh_entries_per_chunk(int prec) { return cz::entries_per_chunk(prec); }
 int czh_write(const char *path, int prec, const int bpd[3], int wtype, double threshold, int encoder,
               const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads)
 {
-    if (!path || !bpd || !records || !offsets || (prec != 4 && prec != 8)) return CZH_EINVAL;
+    return czh_write_subdomains(path, prec, bpd, bpd, wtype, threshold, encoder, records, offsets, nblocks, nthreads);
+}
+
+int czh_write_subdomains(const char *path, int prec, const int bpd[3], const int sub_bpd[3], int wtype,
+                         double threshold, int encoder, const uint8_t *records, const uint64_t *offsets, int nblocks,
+                         int nthreads)
+{
+    if (!path || !bpd || !sub_bpd || !records || !offsets || (prec != 4 && prec != 8)) return CZH_EINVAL;
     cz::Params p;
     p.prec = prec;
-    for (int d = 0; d < 3; ++d) p.bpd[d] = p.sub_bpd[d] = bpd[d];
+    for (int d = 0; d < 3; ++d) {
+        p.bpd[d] = bpd[d];
+        p.sub_bpd[d] = sub_bpd[d];
+    }
     p.wtype = wtype;
     p.threshold = threshold;
     p.encoder = encoder;

@@ -1,14 +1,14 @@
 // cz_file.hpp -- host side of the .cz container: chunked zlib stage, ASCII
 // header, block metadata and chunk look-up table.
 //
-// File layout (one writer, SerializerIO_WaveletCompression_MPI_Simple.h:347-466,
-// 489-591):
+// File layout (SerializerIO_WaveletCompression_MPI_Simple.h:347-466, 489-591):
 //   @0    uint64 header displacement
 //   @8    "\n==============START-BINARY-OCEAN==============\n"      (48 bytes)
-//   @56   per subdomain: its zlib chunks, then uint64 lut[nchunks] holding each
-//         chunk's start relative to the subdomain's first byte
+//   @56   per subdomain (writer rank), in rank order: its chunks, then
+//         uint64 lut[nchunks] holding each chunk's start relative to the
+//         subdomain's first byte
 //   @hd   ASCII header (SerializerIO_..._MPI_Simple.h:515-553)
-//         BlockMetadata[NBLOCKS]                                 (20 bytes each)
+//         BlockMetadata[NBLOCKS], subdomain by subdomain          (20 bytes each)
 //         "\n==============START-BINARY-LUT==============\n"       (46 bytes)
 //         HeaderLUT[SUBDOMAINS]                                  (12 bytes each)
 // A chunk is zlib (RFC 1950, level 6) of consecutive block records
@@ -80,15 +80,29 @@ bool lz4_decompress(const uint8_t *src, size_t n, std::vector<uint8_t> &dst, siz
 // Bytes of one writer chunk's records at most (ENTRIES records of a full block).
 size_t chunk_bound(int prec);
 
-// Serialize one subdomain's record stream (block order, x fastest) into a
-// complete single-subdomain .cz image.  records/offsets as in czw.h.
+// bpd and sub_bpd are positive and each sub_bpd[d] divides bpd[d].
+bool subdomains_valid(const Params &p);
+// The file order of the blocks when p.bpd is cut into subdomains of
+// p.sub_bpd blocks, one writer rank each: order[q] is the block id
+// (ix + bx*(iy + by*iz)) of the q-th block in the file.  Ranks follow MPI's
+// Cartesian order (last coordinate fastest); each rank's blocks are x
+// fastest.  The identity for a single subdomain.
+void file_block_order(const Params &p, std::vector<int> &order);
+
+// Serialize a record stream in block order (x fastest over the whole grid)
+// into a complete .cz image, cut into subdomains as p.sub_bpd says (one
+// writer rank each).  records/offsets as in czw.h.
 int build_cz(const Params &p, const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads,
              std::vector<uint8_t> &file);
-// The two halves of build_cz: the chunk stage (zlib per chunk of ENTRIES
-// records, OpenMP over chunks; offsets relative to records) and the file
-// image around the finished chunks.
+// The two halves of build_cz: the chunk stage (the second stage per chunk of
+// ENTRIES records, OpenMP over chunks) and the file image around the finished
+// chunks.  encode_chunks takes the blocks in file order: with order == nullptr
+// the records are already in that order (offsets relative to records);
+// otherwise block q of the file is records[offsets[order[q]] ...].  Each run
+// of per_subdomain blocks (0: all) is one subdomain, whose last chunk may be
+// short.  assemble_cz takes the chunks subdomain by subdomain.
 int encode_chunks(const Params &p, const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads,
-                  std::vector<std::vector<uint8_t>> &chunks);
+                  std::vector<std::vector<uint8_t>> &chunks, const int *order = nullptr, int per_subdomain = 0);
 int assemble_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks,
                 std::vector<uint8_t> &file);
 

@@ -252,8 +252,11 @@ bool decode_cz(const std::string &path, int expect_wtype, Field &f, bool verbose
 }
 
 #if CZ_TOOL == 1
-// hdf2cz (Tools/hdf2cz.cpp, Test_IO_Compressed.h:249-342).  Extra flags:
-// -precision float|double (raw input), -zerobits, -shuffle, -encoder zlib|lz4|none,
+// hdf2cz (Tools/hdf2cz.cpp, Test_IO_Compressed.h:249-342).  As there,
+// -nprocx/y/z set the writer ranks and -bpdx/y/z the blocks of each rank, so
+// the field holds nproc * bpd blocks per dimension; one process writes the
+// file all those ranks would write together.  Extra flags: -precision
+// float|double (raw input), -zerobits, -shuffle, -encoder zlib|lz4|none,
 // -nowavelets (raw blocks, like the reference's default build).
 int tool_main(int argc, char **argv)
 {
@@ -261,19 +264,21 @@ int tool_main(int argc, char **argv)
     const std::string out = a.str("-czfile", "none");
     if (a.has("-help") || out == "none" || (!a.has("-h5file") && !a.has("-rawfile"))) {
         std::printf("Usage: %s -h5file <hdf5 file> | -rawfile <raw file> -czfile <cz file> -threshold <e> [-wtype <wt>] "
-                    "[-bpdx <nbx>] [-bpdy <nby>] [-bpdz <nbz>] [-precision float|double] [-zerobits <z>] "
+                    "[-bpdx <nbx>] [-bpdy <nby>] [-bpdz <nbz>] [-nprocx <px>] [-nprocy <py>] [-nprocz <pz>] "
+                    "[-precision float|double] [-zerobits <z>] "
                     "[-shuffle] [-encoder zlib|lz4|none] [-nowavelets]\n", "hdf2cz");
         return 1;
     }
-    if (a.i("-nprocx", 1) * a.i("-nprocy", 1) * a.i("-nprocz", 1) != 1) {
-        std::fprintf(stderr, "hdf2cz: one process handles the whole field here (-nproc* must be 1)\n");
-        return 1;
-    }
+    const int nproc[3] = {a.i("-nprocx", 1), a.i("-nprocy", 1), a.i("-nprocz", 1)};
+    const int sub[3] = {a.i("-bpdx", 1), a.i("-bpdy", 1), a.i("-bpdz", 1)};
+    for (int d = 0; d < 3; ++d)
+        if (nproc[d] < 1 || sub[d] < 1) {
+            std::fprintf(stderr, "hdf2cz: -nproc* and -bpd* must be positive\n");
+            return 1;
+        }
     Field f;
     f.prec = a.str("-precision", "float") == "double" ? 8 : 4;
-    f.n[0] = a.i("-bpdx", 1) * 32;
-    f.n[1] = a.i("-bpdy", 1) * 32;
-    f.n[2] = a.i("-bpdz", 1) * 32;
+    for (int d = 0; d < 3; ++d) f.n[d] = nproc[d] * sub[d] * 32;
     if (!load_field(a, f)) return 1;
     const int wtype = a.has("-nowavelets") ? -1 : a.i("-wtype", 3);
     const double threshold = a.f("-threshold", 0.0);
@@ -284,7 +289,10 @@ int tool_main(int argc, char **argv)
     const double t0 = now();
     cz::Params p;
     p.prec = f.prec;
-    for (int d = 0; d < 3; ++d) p.bpd[d] = p.sub_bpd[d] = f.n[d] / 32;
+    for (int d = 0; d < 3; ++d) {
+        p.bpd[d] = f.n[d] / 32;
+        p.sub_bpd[d] = sub[d];
+    }
     p.wtype = wtype;
     p.threshold = threshold;
     {
@@ -322,6 +330,35 @@ int tool_main(int argc, char **argv)
             return 1;
         }
         file_size = file.size();
+    } else if (nproc[0] * nproc[1] * nproc[2] > 1) {
+        // several subdomains: the whole field on the device at once, then the
+        // records regrouped rank by rank on the host
+        void *dfield, *drec, *dws;
+        uint64_t *doff;
+        HIPCHECK(hipMalloc(&dfield, f.data.size()));
+        HIPCHECK(hipMalloc(&drec, czw_records_bound(nb, f.prec)));
+        HIPCHECK(hipMalloc(&dws, czw_compress_workspace_bytes(nb, f.prec)));
+        HIPCHECK(hipMalloc(reinterpret_cast<void **>(&doff), sizeof(uint64_t) * (nb + 1)));
+        HIPCHECK(hipMemcpy(dfield, f.data.data(), f.data.size(), hipMemcpyHostToDevice));
+        const int rc = czw_compress_field(dfield, f.n[0], f.n[1], f.n[2], f.prec, (float)threshold, wtype, zerobits,
+                                          shuffle, drec, doff, dws, nullptr);
+        std::vector<uint64_t> off(nb + 1);
+        std::vector<uint8_t> rec;
+        if (rc == CZW_OK) {
+            HIPCHECK(hipMemcpy(off.data(), doff, off.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
+            rec.resize(off[nb]);
+            HIPCHECK(hipMemcpy(rec.data(), drec, off[nb], hipMemcpyDeviceToHost));
+        }
+        (void)hipFree(dfield);
+        (void)hipFree(drec);
+        (void)hipFree(dws);
+        (void)hipFree(doff);
+        std::vector<uint8_t> file;
+        if (rc != CZW_OK || cz::build_cz(p, rec.data(), off.data(), nb, 0, file) || !cz::write_file(path, file)) {
+            std::fprintf(stderr, "hdf2cz: compressing to %s failed\n", path.c_str());
+            return 1;
+        }
+        file_size = file.size();
     } else {
         // device compressor, zlib on host threads overlapped slab by slab
         void *dfield;
@@ -338,8 +375,9 @@ int tool_main(int argc, char **argv)
         file_size = fb;
     }
     const double t1 = now();
+    const int nsub = nproc[0] * nproc[1] * nproc[2];
     const size_t written = file_size - (cz::make_header(p).size() + nb * sizeof(cz::BlockMetadata) +
-                                          std::strlen(cz::LUT_TITLE) + sizeof(cz::HeaderLUT) + 56);
+                                          std::strlen(cz::LUT_TITLE) + nsub * sizeof(cz::HeaderLUT) + 56);
     std::printf("Channel %d: %.2f kB, wavelet-threshold: %.1e, compr. rate: %.2f\n", 0, written / 1024., threshold,
                 32768.0 * f.prec * nb / (float)written);
     std::printf("done\nElapsed time: %g seconds\n", t1 - t0);

@@ -32,6 +32,7 @@ def load_library():
         vp, i = C.c_void_p, C.c_int
         L.czh_entries_per_chunk.argtypes = [i]
         L.czh_write.argtypes = [C.c_char_p, i, vp, i, C.c_double, i, vp, vp, i, i]
+        L.czh_write_subdomains.argtypes = [C.c_char_p, i, vp, vp, i, C.c_double, i, vp, vp, i, i]
         L.czh_open.restype = vp
         L.czh_open.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t]
         L.czh_close.argtypes = [vp]
@@ -53,18 +54,22 @@ def entries_per_chunk(prec):
     return load_library().czh_entries_per_chunk(int(prec))
 
 
-def write(path, records, offsets, bpd, prec, wtype, threshold, encoder="zlib", nthreads=0):
-    """Write a single-subdomain .cz from a host record stream in block order."""
+def write(path, records, offsets, bpd, prec, wtype, threshold, encoder="zlib", nthreads=0, sub_bpd=None):
+    """Write a .cz from a host record stream in block order (x fastest over
+    the whole grid).  sub_bpd cuts the grid into subdomains the way that many
+    writer ranks would (MPI Cartesian order); None writes one subdomain."""
     rec = np.ascontiguousarray(records, dtype=np.uint8)
     off = np.ascontiguousarray(offsets, dtype=np.uint64)
     b = np.ascontiguousarray(bpd, dtype=np.int32)
+    s = np.ascontiguousarray(bpd if sub_bpd is None else sub_bpd, dtype=np.int32)
     nblocks = int(b.prod())
     if off.size != nblocks + 1:
         raise CzFileError("offsets must have nblocks+1 entries")
-    rc = load_library().czh_write(os.fsencode(path), int(prec), b.ctypes.data, int(wtype), float(threshold),
-                                  ENCODERS[encoder], rec.ctypes.data, off.ctypes.data, nblocks, int(nthreads))
+    rc = load_library().czh_write_subdomains(os.fsencode(path), int(prec), b.ctypes.data, s.ctypes.data, int(wtype),
+                                             float(threshold), ENCODERS[encoder], rec.ctypes.data, off.ctypes.data,
+                                             nblocks, int(nthreads))
     if rc:
-        raise CzFileError(f"czh_write failed ({rc})")
+        raise CzFileError(f"czh_write_subdomains failed ({rc})")
 
 
 def compress_device_field(path, field, eps, wtype=3, zerobits=0, shuffle=False, encoder="zlib", nthreads=0,

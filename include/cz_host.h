@@ -39,6 +39,17 @@ int czh_entries_per_chunk(int prec);
 int czh_write(const char *path, int prec, const int bpd[3], int wtype, double threshold, int encoder,
               const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads);
 
+/* The file that nranks = prod(bpd[d] / sub_bpd[d]) writer ranks, each holding
+ * a subdomain of sub_bpd blocks, write together (the MPI writer of
+ * SerializerIO_WaveletCompression_MPI_Simple.h:347-466 with MPI_Exscan'd
+ * offsets; Tools/Test_IO_Compressed.h:236-291 with -nprocx/y/z).  The records
+ * come in block order of the whole grid, as for czh_write; ranks are placed
+ * in MPI's Cartesian order (last coordinate fastest) and each one chunks its
+ * own blocks, x fastest.  sub_bpd == bpd is czh_write. */
+int czh_write_subdomains(const char *path, int prec, const int bpd[3], const int sub_bpd[3], int wtype,
+                         double threshold, int encoder, const uint8_t *records, const uint64_t *offsets, int nblocks,
+                         int nthreads);
+
 /* Read and parse a .cz file (any number of subdomains).  Returns NULL on
  * error with a message in errbuf. */
 czh_file *czh_open(const char *path, char *errbuf, size_t errlen);

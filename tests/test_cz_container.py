@@ -292,3 +292,83 @@ def test_writer_rejects_unknown_encoder(host, tmp_path):
         czfile.ENCODERS.clear()
         czfile.ENCODERS.update(old)
     assert not (tmp_path / "x.cz").exists()
+
+
+def _ranks_layout_by_hand(pays, bpd, sub, prec, header):
+    """The file that prod(bpd/sub) MPI writer ranks produce together, put
+    together from SerializerIO_WaveletCompression_MPI_Simple.h:347-466 (each
+    rank's chunks and u64 LUT at its MPI_Exscan offset, metadata at
+    rank * NBLOCKS, one HeaderLUT per rank), Test_IO_Compressed.h:57-75 and
+    GridMPI.h:73-114 (Cartesian ranks, last coordinate fastest; blocks x
+    fastest inside a rank, global index = local + coordinate * sub)."""
+    E = czfile.entries_per_chunk(prec)
+    P = [b // s for b, s in zip(bpd, sub)]
+    oceans, metas, hls = [], [], []
+    for cx in range(P[0]):
+        for cy in range(P[1]):
+            for cz in range(P[2]):
+                gids = []
+                for lz in range(sub[2]):
+                    for ly in range(sub[1]):
+                        for lx in range(sub[0]):
+                            gx, gy, gz = cx * sub[0] + lx, cy * sub[1] + ly, cz * sub[2] + lz
+                            gids.append((gx, gy, gz, gx + bpd[0] * (gy + bpd[1] * gz)))
+                chunks = []
+                for c0 in range(0, len(gids), E):
+                    recs = b"".join(struct.pack("<i", len(pays[g[3]])) + pays[g[3]] for g in gids[c0:c0 + E])
+                    chunks.append(czfile.deflate(recs))
+                    for k, g in enumerate(gids[c0:c0 + E]):
+                        metas.append(struct.pack("<5i", c0 // E, k, g[0], g[1], g[2]))
+                lut, pos = [], 0
+                for ch in chunks:
+                    lut.append(pos)
+                    pos += len(ch)
+                ocean = b"".join(chunks) + struct.pack(f"<{len(lut)}Q", *lut)
+                oceans.append(ocean)
+                hls.append(struct.pack("<Qi", len(ocean), len(chunks)))
+    body = b"".join(oceans)
+    return (struct.pack("<Q", 56 + len(body)) + b"\n==============START-BINARY-OCEAN==============\n" + body +
+            header + b"".join(metas) + b"\n==============START-BINARY-LUT==============\n" + b"".join(hls))
+
+
+@pytest.mark.parametrize("prec,bpd,sub", [
+    (4, (2, 2, 1), (1, 2, 1)),
+    (8, (4, 2, 3), (2, 1, 3)),
+    (8, (4, 2, 3), (2, 2, 3)),   # two chunks per rank, the second short
+    (4, (2, 2, 2), (1, 1, 1)),
+    (4, (3, 2, 2), (3, 2, 2)),   # one rank: czh_write's file
+])
+def test_multi_subdomain_writer(oracle, host, tmp_path, prec, bpd, sub):
+    nb = bpd[0] * bpd[1] * bpd[2]
+    rng = np.random.default_rng(5)
+    dt = np.float32 if prec == 4 else np.float64
+    blocks = (rng.standard_normal((nb, 32, 32, 32)) * 1e-2).astype(dt)
+    pays = [oracle.compress(b, 1e-3, 3).tobytes() for b in blocks]
+    rec = np.frombuffer(b"".join(struct.pack("<i", len(p)) + p for p in pays), np.uint8)
+    off = np.cumsum([0] + [4 + len(p) for p in pays]).astype(np.uint64)
+    path = tmp_path / "multi.cz"
+    czfile.write(str(path), rec, off, bpd, prec, 3, 1e-3, sub_bpd=sub)
+    got = path.read_bytes()
+    hd = struct.unpack_from("<Q", got, 0)[0]
+    header = got[hd:got.index(b"START-BINARY-METABLOCKS==============\n", hd) + 38]
+    assert f"SubdomainBlocks: {sub[0]} x {sub[1]} x {sub[2]}\n".encode() in header
+    assert f"Blocks: {bpd[0]} x {bpd[1]} x {bpd[2]}\n".encode() in header
+    assert got == _ranks_layout_by_hand(pays, bpd, sub, prec, header)
+    with czfile.CzFile(str(path)) as f:
+        assert f.bpd == tuple(bpd) and f.sub_bpd == tuple(sub)
+        r2, o2 = f.records()
+    for g in range(nb):
+        o = int(o2[g])
+        n = struct.unpack_from("<i", r2.tobytes(), o)[0]
+        assert r2[o + 4:o + 4 + n].tobytes() == pays[g]
+    if sub == bpd:
+        one = tmp_path / "one.cz"
+        czfile.write(str(one), rec, off, bpd, prec, 3, 1e-3)
+        assert one.read_bytes() == got
+
+
+def test_multi_subdomain_writer_rejects_bad_split(host, tmp_path):
+    rec, off = czfile.records_from_slots(np.zeros((4, 32768 * 4), np.uint8), np.full(4, 32768 * 4))
+    for sub in ((3, 1, 1), (0, 1, 1), (1, 1, 2)):
+        with pytest.raises(czfile.CzFileError):
+            czfile.write(str(tmp_path / "bad.cz"), rec, off, (2, 2, 1), 4, -1, 0.0, sub_bpd=sub)

@@ -92,3 +92,29 @@ def test_lz4_encoder_tools_roundtrip(demo_raw, tmp_path):
     out = _tool("cz2diff", "-czfile1", str(tmp_path / "l.cz"), "-wtype", "3", "-rawfile2", str(demo_raw))
     res = [l for l in out.splitlines() if l.startswith("RES:")][-1].split()
     assert res[2:7] == PUBLISHED_F32.split()[2:7], out
+
+
+def test_multi_rank_files(demo_raw, tmp_path):
+    """The 2x2x2 process grid of Tests/Test1/genref.sh:27 (-nprocx 2 -nprocy 2
+    -nprocz 2 -bpdx 2 -bpdy 2 -bpdz 2): eight subdomains in one file, written
+    by one process.  The wavelet file compared with the uncompressed one gives
+    the published error columns, and holds the payloads of the one-rank file."""
+    grid = ["-nprocx", "2", "-nprocy", "2", "-nprocz", "2", "-bpdx", "2", "-bpdy", "2", "-bpdz", "2"]
+    _tool("hdf2cz", "-rawfile", str(demo_raw), *grid, "-czfile", "ref8.cz", "-fpath", str(tmp_path), "-nowavelets",
+          "-encoder", "none")
+    _tool("hdf2cz", "-rawfile", str(demo_raw), *grid, "-czfile", "tmp8.cz", "-fpath", str(tmp_path),
+          "-threshold", "0.00005", "-wtype", "3")
+    _tool("hdf2cz", "-rawfile", str(demo_raw), "-bpdx", "4", "-bpdy", "4", "-bpdz", "4", "-czfile", "tmp1.cz",
+          "-fpath", str(tmp_path), "-threshold", "0.00005", "-wtype", "3")
+    out = _tool("cz2diff", "-czfile1", str(tmp_path / "tmp8.cz"), "-wtype", "3", "-czfile2", str(tmp_path / "ref8.cz"))
+    res = [l for l in out.splitlines() if l.startswith("RES:")][-1].split()
+    assert res[2:7] == PUBLISHED_F32.split()[2:7], out
+    recs = []
+    for name in ("tmp8.cz", "tmp1.cz"):
+        with czfile.CzFile(str(tmp_path / name)) as f:
+            assert f.bpd == (4, 4, 4) and f.sub_bpd == ((2, 2, 2) if name == "tmp8.cz" else (4, 4, 4))
+            recs.append(f.records())
+    (r8, o8), (r1, o1) = recs
+    for b in range(64):
+        n = int(r1[int(o1[b]):int(o1[b]) + 4].view(np.int32)[0])
+        assert np.array_equal(r8[int(o8[b]):int(o8[b]) + 4 + n], r1[int(o1[b]):int(o1[b]) + 4 + n]), b
