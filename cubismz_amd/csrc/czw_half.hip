@@ -532,17 +532,24 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
             if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_POPCOUNT);
             continue;
         }
-        {  // corner positions: survivor or +0
+        {  // corner positions: survivor or +0; all eight loads are issued
+            // before the first LDS store, so that they overlap
             uint32_t idx = cnt[t];
-            corner_piece<RPD, SPD>(t, [&](int q, int h, int addr, int stride, int, int) {
+            R v[8];
+            corner_piece<RPD, SPD>(t, [&](int q, int h, int, int, int, int) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     if (i < h) {
                         const bool bit = (keep >> (q * h + i)) & 1u;
-                        H[addr + i * stride] = bit ? pay.get(idx) : (R)0;
+                        v[q * h + i] = bit ? pay.get(idx) : (R)0;
                         idx += bit;
                     }
                 }
+            });
+            corner_piece<RPD, SPD>(t, [&](int q, int h, int addr, int stride, int, int) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (i < h) H[addr + i * stride] = v[q * h + i];
             });
         }
         __syncthreads();
