@@ -381,38 +381,109 @@ int scan_records(const uint32_t *nbytes, int n, uint64_t *offsets, hipStream_t s
 // (SerializerIO_WaveletCompression_MPI_Simple.h:237-240): size word, mask,
 // corner survivors then level-32 survivors, with the byte-plane shuffle
 // (WaveletCompressor.cpp:56-73: byte p of survivor i at p*S + i) if asked.
-__global__ __launch_bounds__(256) void k_pack_records(const uint8_t *__restrict__ slots, size_t slot_bytes,
-                                                      const uint32_t *__restrict__ nbytes, int nblocks, int shuffle,
-                                                      const uint64_t *__restrict__ offsets,
-                                                      uint8_t *__restrict__ records)
+// PK_B blocks per workgroup, the first PK_W survivor words of each block
+// loaded before any store.  With fused != 0 the workgroup finds its records'
+// offsets itself (sum of the sizes before it: O(nblocks^2) cached reads in
+// all, used for up to PK_FUSED_MAX blocks) and writes offsets[]; otherwise
+// offsets[] comes from k_scan_records.
+constexpr int PK_T = 512, PK_W = 4, PK_B = 2, PK_FUSED_MAX = 4096;
+__global__ __launch_bounds__(PK_T) void k_pack_records(const uint8_t *__restrict__ slots, size_t slot_bytes,
+                                                       const uint32_t *__restrict__ nbytes, int nblocks, int shuffle,
+                                                       int fused, uint64_t *__restrict__ offsets,
+                                                       uint8_t *__restrict__ records)
 {
-    const int b = blockIdx.x, t = threadIdx.x;
-    const uint32_t n = nbytes[b], sc = nbytes[nblocks + b];
-    const uint32_t S = (n - MASK_BYTES) / 4;
-    uint32_t *dst = reinterpret_cast<uint32_t *>(records + offsets[b]);
-    uint8_t *dst8 = records + offsets[b] + 4 + MASK_BYTES;
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(slots + (size_t)b * slot_bytes);
-    const uint32_t *corner = src + MASK_BYTES / 4, *l32 = corner + CORNER_MAX;
-    if (t == 0) dst[0] = n;
-    for (uint32_t i = t; i < MASK_BYTES / 4; i += 256) dst[1 + i] = src[i];
-    for (uint32_t base = t; base < S; base += 256 * 4) {
-        uint32_t v[4];
+    __shared__ uint64_t wsum[PK_T / 64];
+    __shared__ uint64_t start[PK_B];
+    const int b0 = blockIdx.x * PK_B, t = threadIdx.x;
+    const int nbk = min(PK_B, nblocks - b0);
+    if (fused) {
+        uint64_t part = 0;
+        for (int c = 0; c < b0; c += 8 * PK_T) {
+            uint32_t q[8];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t i = base + 256 * k;
-            v[k] = i < S ? (i < sc ? corner[i] : l32[i - sc]) : 0u;
+            for (int k = 0; k < 8; ++k) {
+                const int i = c + t + PK_T * k;
+                q[k] = i < b0 ? nbytes[i] + 4u : 0u;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) part += q[k];
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t i = base + 256 * k;
-            if (i < S) {
+        for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);
+        if ((t & 63) == 0) wsum[t >> 6] = part;
+        __syncthreads();
+        if (t == 0) {
+            uint64_t o = 0;
+#pragma unroll
+            for (int k = 0; k < PK_T / 64; ++k) o += wsum[k];
+            for (int k = 0; k < nbk; ++k) {
+                start[k] = o;
+                offsets[b0 + k] = o;
+                o += (uint64_t)nbytes[b0 + k] + 4u;
+            }
+            if (b0 + nbk == nblocks) offsets[nblocks] = o;
+        }
+    } else if (t < nbk) {
+        start[t] = offsets[b0 + t];
+    }
+    __syncthreads();
+    constexpr int MW = 1024 / PK_T;
+    uint32_t m[PK_B][MW], v[PK_B][PK_W];
+#pragma unroll
+    for (int kb = 0; kb < PK_B; ++kb) {
+        if (kb < nbk) {
+            const int b = b0 + kb;
+            const uint32_t S = (nbytes[b] - MASK_BYTES) / 4, sc = nbytes[nblocks + b];
+            const uint32_t *src = reinterpret_cast<const uint32_t *>(slots + (size_t)b * slot_bytes);
+            const uint32_t *corner = src + MASK_BYTES / 4, *l32 = corner + CORNER_MAX;
+#pragma unroll
+            for (int k = 0; k < MW; ++k) m[kb][k] = src[t + PK_T * k];
+#pragma unroll
+            for (int k = 0; k < PK_W; ++k) {
+                const uint32_t i = t + PK_T * k;
+                v[kb][k] = i < S ? (i < sc ? corner[i] : l32[i - sc]) : 0u;
+            }
+        }
+    }
+#pragma unroll
+    for (int kb = 0; kb < PK_B; ++kb) {
+        if (kb < nbk) {
+            const int b = b0 + kb;
+            const uint32_t n = nbytes[b];
+            const uint32_t S = (n - MASK_BYTES) / 4, sc = nbytes[nblocks + b];
+            uint32_t *dst = reinterpret_cast<uint32_t *>(records + start[kb]);
+            uint8_t *dst8 = records + start[kb] + 4 + MASK_BYTES;
+            const uint32_t *src = reinterpret_cast<const uint32_t *>(slots + (size_t)b * slot_bytes);
+            const uint32_t *corner = src + MASK_BYTES / 4, *l32 = corner + CORNER_MAX;
+            auto put = [&](uint32_t i, uint32_t w) {
                 if (shuffle) {
-                    dst8[i] = (uint8_t)v[k];
-                    dst8[S + i] = (uint8_t)(v[k] >> 8);
-                    dst8[2 * S + i] = (uint8_t)(v[k] >> 16);
-                    dst8[3 * S + i] = (uint8_t)(v[k] >> 24);
+                    dst8[i] = (uint8_t)w;
+                    dst8[S + i] = (uint8_t)(w >> 8);
+                    dst8[2 * S + i] = (uint8_t)(w >> 16);
+                    dst8[3 * S + i] = (uint8_t)(w >> 24);
                 } else {
-                    dst[1 + MASK_BYTES / 4 + i] = v[k];
+                    dst[1 + MASK_BYTES / 4 + i] = w;
+                }
+            };
+            if (t == 0) dst[0] = n;
+#pragma unroll
+            for (int k = 0; k < MW; ++k) dst[1 + t + PK_T * k] = m[kb][k];
+#pragma unroll
+            for (int k = 0; k < PK_W; ++k) {
+                const uint32_t i = t + PK_T * k;
+                if (i < S) put(i, v[kb][k]);
+            }
+            for (uint32_t c0 = PK_T * PK_W; c0 < S; c0 += PK_T * PK_W) {
+                uint32_t w[PK_W];
+#pragma unroll
+                for (int k = 0; k < PK_W; ++k) {
+                    const uint32_t i = c0 + t + PK_T * k;
+                    w[k] = i < S ? (i < sc ? corner[i] : l32[i - sc]) : 0u;
+                }
+#pragma unroll
+                for (int k = 0; k < PK_W; ++k) {
+                    const uint32_t i = c0 + t + PK_T * k;
+                    if (i < S) put(i, w[k]);
                 }
             }
         }
@@ -445,9 +516,10 @@ static int pack_records(const uint8_t *slots, size_t slot_bytes, const uint32_t 
                         uint8_t *records, uint64_t *offsets, hipStream_t s)
 {
     if (nblocks <= 0 || !slots || !nb || !records || !offsets || (slot_bytes & 3)) return CZW_EINVAL;
-    if (scan_records(nb, nblocks, offsets, s) != CZW_OK) return CZW_EHIP;
-    hipLaunchKernelGGL(k_pack_records, dim3(nblocks), dim3(256), 0, s, slots, slot_bytes, nb, nblocks, shuffle, offsets,
-                       records);
+    const int fused = nblocks <= PK_FUSED_MAX;
+    if (!fused && scan_records(nb, nblocks, offsets, s) != CZW_OK) return CZW_EHIP;
+    hipLaunchKernelGGL(k_pack_records, dim3((nblocks + PK_B - 1) / PK_B), dim3(PK_T), 0, s, slots, slot_bytes, nb,
+                       nblocks, shuffle, fused, offsets, records);
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
 
