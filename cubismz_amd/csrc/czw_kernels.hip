@@ -28,6 +28,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "czw_common.hpp"
 
@@ -81,30 +82,55 @@ CZW_DI int xrow_lds_base(int q)
     else return (q & 31) * SP + (q >> 5) * RP;
 }
 
+// The CR x-rows of thread t.  Block layout: q = t + CT*k.  Field layouts:
+// the rows (y, z0) and (y, z0 + 1) with y = t >> 4, z0 = 2*(t & 15), so that
+// one 8-byte load reads both rows' values at one x (half the load
+// instructions of 4-byte loads; their issue, not their bytes, is what a
+// block's loads cost).
 template <int LAYOUT>
-CZW_DI void load_xrow_f32(const float *__restrict__ src, const FieldGeom &g, int b, int q, float *r)
+CZW_DI int xrow_of(int t, int k)
 {
+    if constexpr (LAYOUT == LAYOUT_BLOCKS) return t + CT * k;
+    else return ((t >> 4) << 5) | (2 * (t & 15) + k);
+}
+
+// x in [X0, X1) of thread t's CR x-rows into r[k][X0 .. X1).
+template <int LAYOUT, int X0, int X1>
+CZW_DI void load_xrows_f32(const float *__restrict__ src, const FieldGeom &g, int b, int t, float (&r)[CR][32])
+{
+    static_assert(CR == 2, "x-rows come in pairs");
     if constexpr (LAYOUT == LAYOUT_BLOCKS) {
-        const float4 *p = reinterpret_cast<const float4 *>(src + (size_t)b * NB3 + (size_t)q * 32);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const float4 v = p[i];
-            r[4 * i] = v.x;
-            r[4 * i + 1] = v.y;
-            r[4 * i + 2] = v.z;
-            r[4 * i + 3] = v.w;
+        for (int k = 0; k < CR; ++k) {
+            const float4 *p = reinterpret_cast<const float4 *>(src + (size_t)b * NB3 + (size_t)(t + CT * k) * 32);
+#pragma unroll
+            for (int i = X0 / 4; i < X1 / 4; ++i) {
+                const float4 v = p[i];
+                r[k][4 * i] = v.x;
+                r[k][4 * i + 1] = v.y;
+                r[k][4 * i + 2] = v.z;
+                r[k][4 * i + 3] = v.w;
+            }
         }
     } else if constexpr (LAYOUT == LAYOUT_FIELD) {
         size_t xs;
         const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(src + field_row(g, b, 0, 0, &xs));
-        const int vo = ((q >> 5) * g.nz + (q & 31)) * 4, so = (int)xs * 4;
+        const int vo = ((t >> 4) * g.nz + 2 * (t & 15)) * 4, so = (int)xs * 4;
 #pragma unroll
-        for (int x = 0; x < 32; ++x) r[x] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, x * so, 0));
+        for (int x = X0; x < X1; ++x) {
+            const czw_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, vo, x * so, 0);
+            r[0][x] = __uint_as_float(v.x);
+            r[1][x] = __uint_as_float(v.y);
+        }
     } else {
         size_t xs;
-        const float *p = src + field_row(g, b, q >> 5, q & 31, &xs);
+        const float *p = src + field_row(g, b, t >> 4, 2 * (t & 15), &xs);
 #pragma unroll
-        for (int x = 0; x < 32; ++x) r[x] = p[x * xs];
+        for (int x = X0; x < X1; ++x) {
+            const float2 v = *reinterpret_cast<const float2 *>(p + x * xs);
+            r[0][x] = v.x;
+            r[1][x] = v.y;
+        }
     }
 }
 
@@ -154,8 +180,7 @@ __global__ __launch_bounds__(CT) void k_compress_f32(const float *__restrict__ s
     if (b < nblocks) {
         const BlockRef<float> br = locate<LAYOUT, float>(T, nf, b, src, nullptr, g, eps);
         eps_b = br.eps;
-#pragma unroll
-        for (int k = 0; k < CR; ++k) load_xrow_f32<RL>(br.src, br.g, br.lb, t + CT * k, nx[k]);
+        load_xrows_f32<RL, 0, 32>(br.src, br.g, br.lb, t, nx);
     }
     for (int it = 0; b < nblocks; ++it) {
         uint8_t *slot = slots + (size_t)b * SLOT32;
@@ -166,28 +191,33 @@ __global__ __launch_bounds__(CT) void k_compress_f32(const float *__restrict__ s
 #pragma unroll
             for (int i = 0; i < 32; ++i) r[i] = nx[k][i];
             row<32, true, WT>(r);
-            lds_st<32>(A, xrow_lds_base<RL>(t + CT * k), 1, r);
+            lds_st<32>(A, xrow_lds_base<RL>(xrow_of<RL>(t, k)), 1, r);
         }
         if (t < 2048 - NRUN32) cnt[NRUN32 + t] = 0u;
         __syncthreads();
-        // Prefetch the next block's rows in two halves, each well under the
-        // 63 outstanding loads a wave may have (more would stall the wave
-        // until the first returns).  The last block reloads itself, so nx is
-        // always written and is dead between its use and its reload.
+        // Prefetch the next block's rows in four quarters of x, issued in
+        // four phases: a CU keeps only some 48 KB of loads in flight, and a
+        // wave that issues past that waits, so 32 KB at a time keeps the
+        // waves computing while the loads run.  The last block reloads
+        // itself, so nx is always written and is dead between its use and
+        // its reload.
         const int bnext = queue ? ticket[it & 1] : b + G;
         const int bn = min(bnext, nblocks - 1);
         const float eps = eps_b;
-        {
+        auto prefetch = [&](auto x0c) {
+            constexpr int X0 = decltype(x0c)::value;
             const BlockRef<float> br = locate<LAYOUT, float>(T, nf, bn, src, nullptr, g, eps);
-            eps_b = br.eps;
-            load_xrow_f32<RL>(br.src, br.g, br.lb, t, nx[0]);
-        }
+            if constexpr (X0 == 0) eps_b = br.eps;
+            load_xrows_f32<RL, X0, X0 + 8>(br.src, br.g, br.lb, t, nx);
+        };
+        prefetch(std::integral_constant<int, 0>());
         {  // level 32, y-pass: rows (z, x)
             int base[CR];
             ypass_bases(t, base);
             pass_rows_f32<true, WT>(A, base, RP);
         }
         __syncthreads();
+        prefetch(std::integral_constant<int, 8>());
         // level 32, z-pass: row (y, x) = physical mask word x + 32*y, bit z
         // (level-32 layout (a0,a1,a2) = (y,x,z)).  Outside the 16^2 corner of
         // (y,x) all 32 results are final; inside it only z >= 16 is, and the
@@ -260,16 +290,25 @@ __global__ __launch_bounds__(CT) void k_compress_f32(const float *__restrict__ s
             uint32_t i0 = (c0 > 0) ? cnt[(c0 - 1) * 256 + base] : 0u;
             uint32_t i1 = cnt[c0 * 256 + base];
             if (staged) {
-                // byte addresses into A; zerobits are applied by the copy-out
+                // Byte addresses into A, each run walked from its last
+                // survivor down: with m = 0 or -1 the keep bit, the target is
+                // (m & a) | (~m & dummy) (v_bfi_b32) and the next a is a + 4*m
+                // (v_lshl_add_u32), three VALU per position with the bit
+                // extract.  Both are written as instructions, as the compiler
+                // otherwise rebuilds the addresses from a running index.
+                // Zerobits are applied by the copy-out.
                 char *Ab = reinterpret_cast<char *>(A);
-                uint32_t a0 = (STAGE32_AT + i0) * 4u, a1 = (STAGE32_AT + i1) * 4u;
+                const uint32_t n0 = __popc(kb[k] & 0xffffu), n1 = __popc(kb[k] >> 16);
+                uint32_t a0 = (STAGE32_AT + i0 + n0 - 1) * 4u, a1 = (STAGE32_AT + i1 + n1 - 1) * 4u;
                 const uint32_t ad = (STAGE32_AT + dummy) * 4u;
 #pragma unroll
-                for (int z = 0; z < 32; ++z) {
-                    const uint32_t bit = (kb[k] >> z) & 1u;
+                for (int z = 31; z >= 0; --z) {
+                    const uint32_t m = (uint32_t)((int32_t)(kb[k] << (31 - z)) >> 31);
                     uint32_t &a = z < 16 ? a0 : a1;
-                    *reinterpret_cast<float *>(Ab + (bit ? a : ad)) = r[k][z];
-                    a += bit << 2;
+                    uint32_t at;
+                    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(at) : "v"(m), "v"(a), "v"(ad));
+                    *reinterpret_cast<float *>(Ab + at) = r[k][z];
+                    asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(a) : "v"(m), "v"(a));
                 }
             } else {
 #pragma unroll
@@ -279,10 +318,7 @@ __global__ __launch_bounds__(CT) void k_compress_f32(const float *__restrict__ s
                 }
             }
         }
-        if constexpr (CR > 1) {  // second half of the prefetch
-            const BlockRef<float> br = locate<LAYOUT, float>(T, nf, bn, src, nullptr, g, eps);
-            load_xrow_f32<RL>(br.src, br.g, br.lb, t + CT, nx[1]);
-        }
+        prefetch(std::integral_constant<int, 16>());
         __syncthreads();  // stage complete, wsum free again
         if (staged) {
             const uint4 *s4 = reinterpret_cast<const uint4 *>(st32);
@@ -298,6 +334,7 @@ __global__ __launch_bounds__(CT) void k_compress_f32(const float *__restrict__ s
             for (uint32_t i = (S32 & ~3u) + t; i < S32; i += CT) d32[i] = st32[i] & zkeep;
         }
         fwt16_8<WT, RP, SP>(A, t);
+        prefetch(std::integral_constant<int, 24>());
 
         // the [0,16)^3 corner: stream positions [0,4096) in pieces of 8 per
         // thread (base 4^3 always kept); values re-read from A
