@@ -364,19 +364,27 @@ def main():
     if int(err.item()):
         raise SystemExit("decoder reported corrupt records")
 
-    # timed region: K steps, per-stage HIP events on the launch stream
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(a.steps)]
+    # timed region: K steps between two HIP events on the launch stream
+    g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    g0.record()
+    for k in range(a.steps):
+        step()
+    g1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_step = g0.elapsed_time(g1) / a.steps * 1e-3
+    ms_per_step = wall / a.steps * 1e3
+    # per-stage times (kernel launch durations for the roofline) from a second
+    # pass of K steps with events between the stages
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(a.steps)]
     for k in range(a.steps):
         step(evs[k])
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     t_c = sum(e[0].elapsed_time(e[1]) for e in evs) / a.steps * 1e-3
     t_p = sum(e[1].elapsed_time(e[2]) for e in evs) / a.steps * 1e-3
     t_d = sum(e[2].elapsed_time(e[3]) for e in evs) / a.steps * 1e-3
-    t_step = sum(e[0].elapsed_time(e[3]) for e in evs) / a.steps * 1e-3
-    ms_per_step = wall / a.steps * 1e3
 
     field_bytes = field.numel() * prec
     other = {}
