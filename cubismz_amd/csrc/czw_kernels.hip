@@ -320,21 +320,61 @@ __global__ __launch_bounds__(CT) void k_compress_f32(const float *__restrict__ s
         }
         prefetch(std::integral_constant<int, 16>());
         __syncthreads();  // stage complete, wsum free again
-        if (staged) {
-            const uint4 *s4 = reinterpret_cast<const uint4 *>(st32);
-            uint4 *d4 = reinterpret_cast<uint4 *>(d32);
-            for (uint32_t i = t; i < S32 / 4; i += CT) {
-                uint4 v = s4[i];
-                v.x &= zkeep;
-                v.y &= zkeep;
-                v.z &= zkeep;
-                v.w &= zkeep;
-                d4[i] = v;
+        // Levels 16 and 8 on the [0,16)^3 corner (fwt16_8's passes) keep 256,
+        // then 64 threads busy.  Meanwhile threads 256.. copy the staged
+        // survivors out (the staging area lies outside the corner), and
+        // every thread issues its last prefetch quarter in a pass where it
+        // has no rows.
+        {
+            float c[16];
+            if (t < 256) {  // level 16 x-pass: row (z, y) in [0,16)^2
+                const int z = t >> 4, y = t & 15;
+                lds_ld<16>(A, z * SP + y * RP, 1, c);
+                row<16, true, WT>(c);
+                lds_st<16>(A, z * SP + y * RP, 1, c);
+            } else {
+                if (staged) {
+                    const uint4 *s4 = reinterpret_cast<const uint4 *>(st32);
+                    uint4 *d4 = reinterpret_cast<uint4 *>(d32);
+                    for (uint32_t i = t - 256; i < S32 / 4; i += CT - 256) {
+                        uint4 v = s4[i];
+                        v.x &= zkeep;
+                        v.y &= zkeep;
+                        v.z &= zkeep;
+                        v.w &= zkeep;
+                        d4[i] = v;
+                    }
+                    for (uint32_t i = (S32 & ~3u) + t - 256; i < S32; i += CT - 256) d32[i] = st32[i] & zkeep;
+                }
+                prefetch(std::integral_constant<int, 24>());
             }
-            for (uint32_t i = (S32 & ~3u) + t; i < S32; i += CT) d32[i] = st32[i] & zkeep;
+            __syncthreads();
+            if (t < 256) {  // level 16 y-pass, then level 8 y-pass: row (z, x)
+                const int z = t >> 4, x = t & 15;
+                lds_ld<16>(A, z * SP + x, RP, c);
+                row<16, true, WT>(c);
+                if (z < 8 && x < 8) row<8, true, WT>(c);
+                lds_st<16>(A, z * SP + x, RP, c);
+            }
+            __syncthreads();
+            if (t < 64) {  // level 8 z-pass: row (y, x) in [0,8)^2
+                const int y = t >> 3, x = t & 7;
+                lds_ld<8>(A, y * RP + x, SP, c);
+                row<8, true, WT>(c);
+                lds_st<8>(A, y * RP + x, SP, c);
+            } else if (t < 256) {
+                prefetch(std::integral_constant<int, 24>());
+            }
+            __syncthreads();
+            if (t < 64) {  // level 8 x-pass: row (z, y)
+                const int z = t >> 3, y = t & 7;
+                lds_ld<8>(A, z * SP + y * RP, 1, c);
+                row<8, true, WT>(c);
+                lds_st<8>(A, z * SP + y * RP, 1, c);
+            }
+            __syncthreads();
+            if (t < 64) prefetch(std::integral_constant<int, 24>());
         }
-        fwt16_8<WT, RP, SP>(A, t);
-        prefetch(std::integral_constant<int, 24>());
 
         // the [0,16)^3 corner: stream positions [0,4096) in pieces of 8 per
         // thread (base 4^3 always kept); values re-read from A
