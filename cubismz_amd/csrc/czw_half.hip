@@ -411,18 +411,18 @@ __global__ __launch_bounds__(256) void k_pack_records_h(const uint8_t *__restric
 // Decompressor (WaveletCompressor.cpp:232-259 with FullWaveletTransform.h:
 // 100-150 and the inverse transform), one block per workgroup iteration.
 // Survivors are read straight from the record (they are at 4-byte offsets;
-// byte planes when shuffled).
-template <typename R>
+// byte planes when shuffled).  SHUF is a template parameter so that the
+// per-position gathers carry no test of it.
+template <typename R, bool SHUF>
 struct Payload {
     const uint32_t *w;  // unshuffled: survivor i = words W*i .. W*i + W - 1
     const uint8_t *b8;  // shuffled: byte p of survivor i at p*S + i
     uint32_t S;
-    int shuffle;
     CZW_DI R get(uint32_t i) const
     {
         if constexpr (sizeof(R) == 8) {
             unsigned long long u;
-            if (shuffle) {
+            if constexpr (SHUF) {
                 u = 0;
 #pragma unroll
                 for (int p = 0; p < 8; ++p) u |= (unsigned long long)b8[(size_t)p * S + i] << (8 * p);
@@ -432,7 +432,7 @@ struct Payload {
             return __longlong_as_double((long long)u);
         } else {
             uint32_t u;
-            if (shuffle) {
+            if constexpr (SHUF) {
                 u = (uint32_t)b8[i] | ((uint32_t)b8[S + i] << 8) | ((uint32_t)b8[2 * (size_t)S + i] << 16) |
                     ((uint32_t)b8[3 * (size_t)S + i] << 24);
             } else {
@@ -443,10 +443,10 @@ struct Payload {
     }
 };
 
-template <typename R, int WT, int LAYOUT>
+template <typename R, int WT, int LAYOUT, bool SHUF>
 __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(const uint8_t *__restrict__ records, uint64_t rec_bytes,
                                                        const uint64_t *__restrict__ offsets, int nblocks,
-                                                       int shuffle, R *__restrict__ dst, FieldGeom g,
+                                                       R *__restrict__ dst, FieldGeom g,
                                                        uint32_t *__restrict__ err,
                                                        const czw_field_t *__restrict__ fields, int nf,
                                                        uint32_t *__restrict__ queue)
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
             continue;
         }
         const uint32_t S = (nb - MASK_BYTES) / (uint32_t)sizeof(R);
-        const Payload<R> pay{p + 1 + MASK_BYTES / 4, records + ob + 4 + MASK_BYTES, S, shuffle};
+        const Payload<R, SHUF> pay{p + 1 + MASK_BYTES / 4, records + ob + 4 + MASK_BYTES, S};
         __syncthreads();  // the previous block's last reads of H, M, cnt and flag are done
         if (t == 0) flag = 1u;
 #pragma unroll
@@ -710,18 +710,31 @@ int pack_records_half(int prec, const uint8_t *slots, const uint32_t *nbytes, in
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
 
+template <typename R, int LAYOUT, bool SHUF>
+static int launch_decompress_hs(int wt, const uint8_t *rec, uint64_t rb, const uint64_t *offsets, int nblocks, R *dst,
+                                FieldGeom g, uint32_t *err, const czw_field_t *fields, int nf, int groups,
+                                uint32_t *queue, hipStream_t s)
+{
+    const void *fn = wt == 1 ? (const void *)k_decompress_h<R, 1, LAYOUT, SHUF>
+                   : wt == 2 ? (const void *)k_decompress_h<R, 2, LAYOUT, SHUF>
+                             : (const void *)k_decompress_h<R, 3, LAYOUT, SHUF>;
+    const dim3 grid(std::min(nblocks, groups ? groups : resident_groups(fn, NT))), blk(NT);
+    if (wt == 1) hipLaunchKernelGGL((k_decompress_h<R, 1, LAYOUT, SHUF>), grid, blk, 0, s, rec, rb, offsets, nblocks, dst, g, err, fields, nf, queue);
+    if (wt == 2) hipLaunchKernelGGL((k_decompress_h<R, 2, LAYOUT, SHUF>), grid, blk, 0, s, rec, rb, offsets, nblocks, dst, g, err, fields, nf, queue);
+    if (wt == 3) hipLaunchKernelGGL((k_decompress_h<R, 3, LAYOUT, SHUF>), grid, blk, 0, s, rec, rb, offsets, nblocks, dst, g, err, fields, nf, queue);
+    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
+}
+
 template <typename R, int LAYOUT>
 static int launch_decompress_h(int wt, const uint8_t *rec, uint64_t rb, const uint64_t *offsets, int nblocks,
                                int shuffle, R *dst, FieldGeom g, uint32_t *err, const czw_field_t *fields, int nf,
                                int groups, uint32_t *queue, hipStream_t s)
 {
-    const void *fn = wt == 1 ? (const void *)k_decompress_h<R, 1, LAYOUT>
-                   : wt == 2 ? (const void *)k_decompress_h<R, 2, LAYOUT> : (const void *)k_decompress_h<R, 3, LAYOUT>;
-    const dim3 grid(std::min(nblocks, groups ? groups : resident_groups(fn, NT))), blk(NT);
-    if (wt == 1) hipLaunchKernelGGL((k_decompress_h<R, 1, LAYOUT>), grid, blk, 0, s, rec, rb, offsets, nblocks, shuffle, dst, g, err, fields, nf, queue);
-    if (wt == 2) hipLaunchKernelGGL((k_decompress_h<R, 2, LAYOUT>), grid, blk, 0, s, rec, rb, offsets, nblocks, shuffle, dst, g, err, fields, nf, queue);
-    if (wt == 3) hipLaunchKernelGGL((k_decompress_h<R, 3, LAYOUT>), grid, blk, 0, s, rec, rb, offsets, nblocks, shuffle, dst, g, err, fields, nf, queue);
-    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
+    if (shuffle)
+        return launch_decompress_hs<R, LAYOUT, true>(wt, rec, rb, offsets, nblocks, dst, g, err, fields, nf, groups,
+                                                     queue, s);
+    return launch_decompress_hs<R, LAYOUT, false>(wt, rec, rb, offsets, nblocks, dst, g, err, fields, nf, groups, queue,
+                                                  s);
 }
 
 template <typename R>
