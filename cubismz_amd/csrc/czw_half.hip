@@ -388,21 +388,48 @@ __global__ __launch_bounds__(256) void k_pack_records_h(const uint8_t *__restric
     uint32_t *dst = reinterpret_cast<uint32_t *>(records + offsets[b]);
     uint8_t *dst8 = records + offsets[b] + 4 + MASK_BYTES;
     if (t == 0) dst[0] = n;
-    for (uint32_t i = t; i < MASK_BYTES / 4; i += 256) dst[1 + i] = reinterpret_cast<const uint32_t *>(slot)[i];
-    for (uint32_t i = t; i < S; i += 256) {
-        int seg = 0;
+    {
+        uint32_t m[MASK_BYTES / 4 / 256];
 #pragma unroll
-        for (int j = 1; j < NCNT; ++j) seg += (i >= st[j]) ? 1 : 0;
-        const size_t e = from[seg] + (i - st[seg]);
-        uint32_t w[W];
+        for (int k = 0; k < MASK_BYTES / 4 / 256; ++k) m[k] = reinterpret_cast<const uint32_t *>(slot)[t + 256 * k];
 #pragma unroll
-        for (int k = 0; k < W; ++k) w[k] = reg[e * W + k];
-        if (shuffle) {
+        for (int k = 0; k < MASK_BYTES / 4 / 256; ++k) dst[1 + t + 256 * k] = m[k];
+    }
+    // U survivors per thread per round, all loaded before the first store
+    // (one memory round trip per round instead of per survivor)
+    constexpr int U = 8;
+    for (uint32_t i0 = t; i0 < S; i0 += 256 * U) {
+        uint32_t w[U][W];
 #pragma unroll
-            for (int p = 0; p < 4 * W; ++p) dst8[(size_t)p * S + i] = (uint8_t)(w[p >> 2] >> (8 * (p & 3)));
-        } else {
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + 256 * u;
+            if (i < S) {
+                int seg = 0;
 #pragma unroll
-            for (int k = 0; k < W; ++k) dst[1 + MASK_BYTES / 4 + (size_t)W * i + k] = w[k];
+                for (int j = 1; j < NCNT; ++j) seg += (i >= st[j]) ? 1 : 0;
+                const size_t e = from[seg] + (i - st[seg]);
+                if constexpr (W == 2) {
+                    const uint2 v = reinterpret_cast<const uint2 *>(reg)[e];
+                    w[u][0] = v.x;
+                    w[u][1] = v.y;
+                } else {
+                    w[u][0] = reg[e];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + 256 * u;
+            if (i < S) {
+                if (shuffle) {
+#pragma unroll
+                    for (int p = 0; p < 4 * W; ++p)
+                        dst8[(size_t)p * S + i] = (uint8_t)(w[u][p >> 2] >> (8 * (p & 3)));
+                } else {
+#pragma unroll
+                    for (int k = 0; k < W; ++k) dst[1 + MASK_BYTES / 4 + (size_t)W * i + k] = w[u][k];
+                }
+            }
         }
     }
 }

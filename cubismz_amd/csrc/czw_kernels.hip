@@ -415,7 +415,7 @@ __global__ __launch_bounds__(CT) void k_compress_f32(const float *__restrict__ s
 
 // Exclusive scan of record sizes (4 + nbytes[b]) into offsets[0..n]; one
 // workgroup of 1024 threads, each owning a contiguous run of ceil(n/1024)
-// blocks so that every load is issued before the scan.
+// blocks, read eight at a time.
 __global__ __launch_bounds__(1024) void k_scan_records(const uint32_t *__restrict__ nbytes, int n,
                                                        uint64_t *__restrict__ offsets)
 {
@@ -424,7 +424,13 @@ __global__ __launch_bounds__(1024) void k_scan_records(const uint32_t *__restric
     const int per = (n + 1023) / 1024;
     const int i0 = min(n, t * per), i1 = min(n, i0 + per);
     uint64_t v = 0;
-    for (int i = i0; i < i1; ++i) v += (uint64_t)nbytes[i] + 4u;
+    for (int c = i0; c < i1; c += 8) {  // eight loads in flight per thread
+        uint32_t q[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[k] = c + k < i1 ? nbytes[c + k] + 4u : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v += q[k];
+    }
     uint64_t inc = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -441,9 +447,15 @@ __global__ __launch_bounds__(1024) void k_scan_records(const uint32_t *__restric
         tot += s;
     }
     run += inc - v;
-    for (int i = i0; i < i1; ++i) {
-        offsets[i] = run;
-        run += (uint64_t)nbytes[i] + 4u;
+    for (int c = i0; c < i1; c += 8) {
+        uint32_t q[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) q[k] = c + k < i1 ? nbytes[c + k] + 4u : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (c + k < i1) offsets[c + k] = run;
+            run += q[k];
+        }
     }
     if (t == 0) offsets[n] = tot;
 }
