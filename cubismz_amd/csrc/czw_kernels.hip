@@ -414,50 +414,48 @@ __global__ __launch_bounds__(CT) void k_compress_f32(const float *__restrict__ s
 }
 
 // Exclusive scan of record sizes (4 + nbytes[b]) into offsets[0..n]; one
-// workgroup of 1024 threads, each owning a contiguous run of ceil(n/1024)
-// blocks, read eight at a time.
+// workgroup of 1024 threads over chunks of 8192 sizes staged in LDS, so that
+// the global loads and the 8-byte offset stores are both coalesced.  A
+// chunk's sizes sum to at most 8192 records of <= 266244 bytes: u32.
 __global__ __launch_bounds__(1024) void k_scan_records(const uint32_t *__restrict__ nbytes, int n,
                                                        uint64_t *__restrict__ offsets)
 {
-    __shared__ uint64_t wsum[16];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int per = (n + 1023) / 1024;
-    const int i0 = min(n, t * per), i1 = min(n, i0 + per);
-    uint64_t v = 0;
-    for (int c = i0; c < i1; c += 8) {  // eight loads in flight per thread
-        uint32_t q[8];
+    constexpr int CH = 8192;
+    __shared__ uint32_t P[CH];
+    __shared__ uint32_t wsum[16];
+    const int t = threadIdx.x;
+    uint64_t carry = 0;
+    for (int c0 = 0; c0 < n; c0 += CH) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) q[k] = c + k < i1 ? nbytes[c + k] + 4u : 0u;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) v += q[k];
-    }
-    uint64_t inc = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint64_t y = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += y;
-    }
-    if (lane == 63) wsum[w] = inc;
-    __syncthreads();
-    uint64_t run = 0, tot = 0;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const uint64_t s = wsum[k];
-        run += (k < w) ? s : 0u;
-        tot += s;
-    }
-    run += inc - v;
-    for (int c = i0; c < i1; c += 8) {
-        uint32_t q[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) q[k] = c + k < i1 ? nbytes[c + k] + 4u : 0u;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            if (c + k < i1) offsets[c + k] = run;
-            run += q[k];
+        for (int j = 0; j < CH / 1024; ++j) {
+            const int i = c0 + t + 1024 * j;
+            P[t + 1024 * j] = i < n ? nbytes[i] + 4u : 0u;
         }
+        __syncthreads();
+        uint32_t v[CH / 1024], sum = 0;
+#pragma unroll
+        for (int j = 0; j < CH / 1024; ++j) {
+            v[j] = P[(CH / 1024) * t + j];
+            sum += v[j];
+        }
+        uint32_t tot;
+        uint32_t run = block_excl_scan<1024>(sum, wsum, &tot);
+        __syncthreads();  // every thread has read its sizes
+#pragma unroll
+        for (int j = 0; j < CH / 1024; ++j) {
+            P[(CH / 1024) * t + j] = run;
+            run += v[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < CH / 1024; ++j) {
+            const int i = c0 + t + 1024 * j;
+            if (i < n) offsets[i] = carry + P[t + 1024 * j];
+        }
+        carry += tot;
+        __syncthreads();  // P and wsum are reused by the next chunk
     }
-    if (t == 0) offsets[n] = tot;
+    if (t == 0) offsets[n] = carry;
 }
 
 int scan_records(const uint32_t *nbytes, int n, uint64_t *offsets, hipStream_t s)
