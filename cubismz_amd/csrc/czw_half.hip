@@ -68,14 +68,14 @@ CZW_DI void load_xrow_h(const R *__restrict__ src, const FieldGeom &g, int b, in
         if constexpr (sizeof(R) == 8) {
 #pragma unroll
             for (int i = 0; i < N / 2; ++i) {
-                const double2 v = reinterpret_cast<const double2 *>(p)[i];
+                const czw_f64x2 v = __builtin_nontemporal_load(reinterpret_cast<const czw_f64x2 *>(p) + i);
                 r[x0 + 2 * i] = v.x;
                 r[x0 + 2 * i + 1] = v.y;
             }
         } else {
 #pragma unroll
             for (int i = 0; i < N / 4; ++i) {
-                const float4 v = reinterpret_cast<const float4 *>(p)[i];
+                const czw_f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const czw_f32x4 *>(p) + i);
                 r[x0 + 4 * i] = v.x;
                 r[x0 + 4 * i + 1] = v.y;
                 r[x0 + 4 * i + 2] = v.z;
@@ -89,15 +89,15 @@ CZW_DI void load_xrow_h(const R *__restrict__ src, const FieldGeom &g, int b, in
 #pragma unroll
         for (int x = x0; x < x0 + N; ++x) {
             if constexpr (sizeof(R) == 8)
-                r[x] = from_bits<double>(__builtin_amdgcn_raw_buffer_load_b64(rs, vo, x * so, 0));
+                r[x] = from_bits<double>(__builtin_amdgcn_raw_buffer_load_b64(rs, vo, x * so, CZW_FIELD_CPOL));
             else
-                r[x] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, x * so, 0));
+                r[x] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, vo, x * so, CZW_FIELD_CPOL));
         }
     } else {
         size_t xs;
         const R *p = src + field_row(g, b, q >> 5, q & 31, &xs);
 #pragma unroll
-        for (int x = x0; x < x0 + N; ++x) r[x] = p[x * xs];
+        for (int x = x0; x < x0 + N; ++x) r[x] = __builtin_nontemporal_load(p + x * xs);
     }
 }
 
@@ -108,11 +108,13 @@ CZW_DI void store_xrow_h(R *__restrict__ dst, const FieldGeom &g, int b, int q, 
         R *p = dst + (size_t)b * NB3 + (size_t)q * 32;
         if constexpr (sizeof(R) == 8) {
 #pragma unroll
-            for (int i = 0; i < 16; ++i) reinterpret_cast<double2 *>(p)[i] = make_double2(r[2 * i], r[2 * i + 1]);
+            for (int i = 0; i < 16; ++i)
+                __builtin_nontemporal_store(czw_f64x2{r[2 * i], r[2 * i + 1]}, reinterpret_cast<czw_f64x2 *>(p) + i);
         } else {
 #pragma unroll
             for (int i = 0; i < 8; ++i)
-                reinterpret_cast<float4 *>(p)[i] = make_float4(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
+                __builtin_nontemporal_store(czw_f32x4{r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]},
+                                            reinterpret_cast<czw_f32x4 *>(p) + i);
         }
     } else if constexpr (RL == LAYOUT_FIELD) {
         size_t xs;
@@ -123,16 +125,16 @@ CZW_DI void store_xrow_h(R *__restrict__ dst, const FieldGeom &g, int b, int q, 
             if constexpr (sizeof(R) == 8) {
                 const unsigned long long u = (unsigned long long)__double_as_longlong(r[x]);
                 const czw_u32x2 v = {(unsigned)u, (unsigned)(u >> 32)};
-                __builtin_amdgcn_raw_buffer_store_b64(v, rs, vo, x * so, 0);
+                __builtin_amdgcn_raw_buffer_store_b64(v, rs, vo, x * so, CZW_FIELD_CPOL);
             } else {
-                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(r[x]), rs, vo, x * so, 0);
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(r[x]), rs, vo, x * so, CZW_FIELD_CPOL);
             }
         }
     } else {
         size_t xs;
         R *p = dst + field_row(g, b, q >> 5, q & 31, &xs);
 #pragma unroll
-        for (int x = 0; x < 32; ++x) p[x * xs] = r[x];
+        for (int x = 0; x < 32; ++x) __builtin_nontemporal_store(r[x], p + x * xs);
     }
 }
 

@@ -102,10 +102,10 @@ CZW_DI void load_xrows_f32(const float *__restrict__ src, const FieldGeom &g, in
     if constexpr (LAYOUT == LAYOUT_BLOCKS) {
 #pragma unroll
         for (int k = 0; k < CR; ++k) {
-            const float4 *p = reinterpret_cast<const float4 *>(src + (size_t)b * NB3 + (size_t)(t + CT * k) * 32);
+            const czw_f32x4 *p = reinterpret_cast<const czw_f32x4 *>(src + (size_t)b * NB3 + (size_t)(t + CT * k) * 32);
 #pragma unroll
             for (int i = X0 / 4; i < X1 / 4; ++i) {
-                const float4 v = p[i];
+                const czw_f32x4 v = __builtin_nontemporal_load(p + i);
                 r[k][4 * i] = v.x;
                 r[k][4 * i + 1] = v.y;
                 r[k][4 * i + 2] = v.z;
@@ -118,7 +118,7 @@ CZW_DI void load_xrows_f32(const float *__restrict__ src, const FieldGeom &g, in
         const int vo = ((t >> 4) * g.nz + 2 * (t & 15)) * 4, so = (int)xs * 4;
 #pragma unroll
         for (int x = X0; x < X1; ++x) {
-            const czw_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, vo, x * so, 0);
+            const czw_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, vo, x * so, CZW_FIELD_CPOL);
             r[0][x] = __uint_as_float(v.x);
             r[1][x] = __uint_as_float(v.y);
         }
@@ -127,7 +127,7 @@ CZW_DI void load_xrows_f32(const float *__restrict__ src, const FieldGeom &g, in
         const float *p = src + field_row(g, b, t >> 4, 2 * (t & 15), &xs);
 #pragma unroll
         for (int x = X0; x < X1; ++x) {
-            const float2 v = *reinterpret_cast<const float2 *>(p + x * xs);
+            const czw_f32x2 v = __builtin_nontemporal_load(reinterpret_cast<const czw_f32x2 *>(p + x * xs));
             r[0][x] = v.x;
             r[1][x] = v.y;
         }
