@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--no-f64", action="store_true", help="skip the binary64 scale-up leg (1024^3 f64)")
     ap.add_argument("--f64-size", type=int, default=1024)
     ap.add_argument("--no-cz", action="store_true", help="skip the end-to-end .cz (host zlib) measurement")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time the headline steps as separate launches instead of one HIP graph per step")
     ap.add_argument("--field-cache", default="",
                     help="raw file for the headline field: loaded when present, written after generation otherwise "
                          "(lets profiler runs skip the generator's kernels)")
@@ -364,18 +366,53 @@ def main():
     if int(err.item()):
         raise SystemExit("decoder reported corrupt records")
 
+    # One HIP graph per step (compress, pack, decompress): the three Python
+    # launches take about as long as the kernels, so launched one by one the
+    # GPU would wait for the host.  The kernels clear nothing with memsets,
+    # so the step captures as it is; the last replay is checked against an
+    # eager step below.
+    graph = None
+    if not a.no_graph:
+        try:
+            cs = torch.cuda.Stream()
+            cs.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(cs):
+                step()
+            torch.cuda.current_stream().wait_stream(cs)
+            torch.cuda.synchronize()
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph, stream=cs, capture_error_mode="relaxed"):
+                step()
+            torch.cuda.synchronize()
+        except Exception as e:  # keep the measurement: fall back to eager launches
+            print(f"[bench] graph capture failed ({e}); timing eager launches", file=sys.stderr, flush=True)
+            graph = None
+            torch.cuda.synchronize()
+
     # timed region: K steps between two HIP events on the launch stream
     g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     g0.record()
     for k in range(a.steps):
-        step()
+        if graph is not None:
+            graph.replay()
+        else:
+            step()
     g1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     t_step = g0.elapsed_time(g1) / a.steps * 1e-3
     ms_per_step = wall / a.steps * 1e3
+    graph_ok = None
+    if graph is not None:
+        rg, og, fg = records.clone(), offsets.clone(), out.clone()
+        step()
+        torch.cuda.synchronize()
+        ne = int(offsets[-1].item())
+        graph_ok = bool(torch.equal(og, offsets) and torch.equal(rg[:ne], records[:ne]) and
+                        torch.equal(fg.view(torch.int32), out.view(torch.int32)) and int(err.item()) == 0)
+        del rg, og, fg
     # per-stage times (kernel launch durations for the roofline) from a second
     # pass of K steps with events between the stages
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(a.steps)]
@@ -477,6 +514,8 @@ def main():
                                f"{nblocks} blocks of 32^3",
                    "eps": a.eps, "wtype": a.wtype, "zerobits": a.zerobits, "shuffle": a.shuffle},
         "stages_ms": {"compress": round(t_c * 1e3, 4), "pack": round(t_p * 1e3, 4), "decompress": round(t_d * 1e3, 4)},
+        "launch": "one HIP graph per step" if graph is not None else "three launches per step",
+        "graph_matches_eager": graph_ok,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "alg_bytes_per_launch": kern[dom][0], "avg_launch_ms": round(kern[dom][1] * 1e3, 4)},
