@@ -118,3 +118,31 @@ def test_multi_rank_files(demo_raw, tmp_path):
     for b in range(64):
         n = int(r1[int(o1[b]):int(o1[b]) + 4].view(np.int32)[0])
         assert np.array_equal(r8[int(o8[b]):int(o8[b]) + 4 + n], r1[int(o1[b]):int(o1[b]) + 4 + n]), b
+
+
+def test_hdf5_write_and_read(demo_raw, tmp_path):
+    """The HDF5 side of the tools (Tools/cz2hdf.cpp:158-199,
+    Test_IO_Compressed.h:137-212): cz2hdf writes /data (NX,NY,NZ,1), and
+    hdf2cz -h5file reads it back; stored uncompressed (-nowavelets -encoder
+    none), the values are those cz2hdf wrote as a raw file."""
+    exe = os.path.join(BIN, "hdf2cz")
+    if os.path.exists(exe) and "hdf5" not in subprocess.run(["ldd", exe], capture_output=True, text=True).stdout:
+        pytest.skip("tools built without HDF5")
+    _tool("hdf2cz", "-rawfile", str(demo_raw), "-bpdx", "4", "-bpdy", "4", "-bpdz", "4", "-czfile", "tmp.cz",
+          "-fpath", str(tmp_path), "-threshold", "0.00005", "-wtype", "3")
+    _tool("cz2hdf", "-czfile", str(tmp_path / "tmp.cz"), "-h5file", str(tmp_path / "out"))
+    _tool("cz2hdf", "-czfile", str(tmp_path / "tmp.cz"), "-rawfile", str(tmp_path / "out.raw"))
+    assert (tmp_path / "out.h5").exists() and (tmp_path / "out.h5.xmf").exists()
+    _tool("hdf2cz", "-h5file", str(tmp_path / "out.h5"), "-bpdx", "4", "-bpdy", "4", "-bpdz", "4", "-czfile", "r.cz",
+          "-fpath", str(tmp_path), "-nowavelets", "-encoder", "none")
+    want = np.fromfile(tmp_path / "out.raw", np.float32).reshape(128, 128, 128)
+    with czfile.CzFile(str(tmp_path / "r.cz")) as f:
+        rec, off = f.records()
+    got = np.empty_like(want)
+    for b in range(64):
+        ix, iy, iz = b % 4, (b // 4) % 4, b // 16
+        o = int(off[b])
+        assert int(rec[o:o + 4].view(np.int32)[0]) == 32768 * 4
+        blk = rec[o + 4:o + 4 + 32768 * 4].view(np.float32).reshape(32, 32, 32)  # [z][y][x]
+        got[32 * ix:32 * ix + 32, 32 * iy:32 * iy + 32, 32 * iz:32 * iz + 32] = blk.transpose(2, 1, 0)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
