@@ -477,7 +477,7 @@ def main():
 
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         threads = O.set_threads(threads)
-        print(f"[bench] cpu baseline (oracle, {threads} threads) + parity over {nblocks} blocks", file=sys.stderr,
+        print(f"[bench] cpu baseline ({threads} threads) + parity over {nblocks} blocks", file=sys.stderr,
               flush=True)
         tc0 = time.perf_counter()
         oslots, onb = O.compress_field(fh, a.eps, a.wtype, a.zerobits, a.shuffle)
@@ -485,8 +485,25 @@ def main():
         ofield = O.decompress_field(oslots, onb, shape, a.wtype, "f32", a.shuffle)
         tc2 = time.perf_counter()
         cpu = {"value": round(field_bytes / (tc2 - tc0) / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
-               "sample": f"whole {n}^3 field ({nblocks} blocks), compress {tc1 - tc0:.3f} s + decompress "
-                         f"{tc2 - tc1:.3f} s"}
+               "sample": f"oracle over the whole {n}^3 field ({nblocks} blocks), compress {tc1 - tc0:.3f} s + "
+                         f"decompress {tc2 - tc1:.3f} s"}
+        if O.ref_available("f32", a.zerobits, bool(a.shuffle)):
+            # the reference's own compressor and decoder (built from its sources
+            # into oracle/_ref), OpenMP over blocks as its writer and reader run
+            reps, t_rc, t_rd = 5, 0.0, 0.0
+            for _ in range(reps):
+                tr0 = time.perf_counter()
+                rsizes = O.ref_field_compress(fh, a.eps, a.wtype, threads, a.zerobits, bool(a.shuffle))
+                tr1 = time.perf_counter()
+                O.ref_field_decompress(recs, off, shape, a.wtype, "f32", threads, a.zerobits, bool(a.shuffle))
+                tr2 = time.perf_counter()
+                t_rc, t_rd = t_rc + (tr1 - tr0), t_rd + (tr2 - tr1)
+            cpu = {"value": round(reps * field_bytes / (t_rc + t_rd) / 1e9, 4), "unit": "GB/s", "cores": threads,
+                   "kind": "reference",
+                   "sample": f"whole {n}^3 field ({nblocks} blocks) {reps} times, compress {t_rc / reps:.3f} s + "
+                             f"decompress {t_rd / reps:.3f} s per pass; the oracle port on the same: "
+                             f"{field_bytes / (tc2 - tc0) / 1e9:.3f} GB/s",
+                   "reference_sizes_match": bool(np.array_equal(rsizes, nb))}
         mism = 0
         for b in range(nblocks):
             o = int(off[b])

@@ -203,6 +203,11 @@ def ref(prec="f32", zerobits=0, shuffle=False):
         L.rf_iwt.argtypes = [C.c_void_p, C.c_int]
         L.rf_row.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.rf_sizeof_compressor.restype = C.c_long
+        L.rf_field_compress.restype = C.c_long
+        L.rf_field_compress.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
+                                        C.c_void_p]
+        L.rf_field_decompress.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.c_void_p]
         _refs[p] = L
     return _refs[p]
 
@@ -263,3 +268,24 @@ def ref_lz4_compress(data):
 
 def ref_lz4_available():
     return os.path.exists(os.path.join(HERE, "_ref", "libczref_lz4.so"))
+
+
+def ref_field_compress(field, eps, wtype, nthreads, zerobits=0, shuffle=False):
+    """The reference's compressor over every block of an HDF5-ordered field
+    (OpenMP over blocks); returns the payload sizes."""
+    f = np.ascontiguousarray(field)
+    prec = "f32" if f.dtype == np.float32 else "f64"
+    sizes = np.zeros(f.size // 32768, np.int64)
+    ref(prec, zerobits, shuffle).rf_field_compress(_ptr(f), f.shape[0], f.shape[1], f.shape[2], float(eps), wtype,
+                                                   int(nthreads), _ptr(sizes))
+    return sizes
+
+
+def ref_field_decompress(records, offsets, shape, wtype, prec, nthreads, zerobits=0, shuffle=False):
+    """The reference's decoder over every record of a stream into a field."""
+    rec = np.ascontiguousarray(records, np.uint8)
+    off = np.ascontiguousarray(offsets, np.int64)
+    out = np.empty(shape, np.float32 if prec == "f32" else np.float64)
+    ref(prec, zerobits, shuffle).rf_field_decompress(_ptr(rec), _ptr(off), shape[0], shape[1], shape[2], wtype,
+                                                     int(nthreads), _ptr(out))
+    return out

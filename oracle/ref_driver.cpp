@@ -70,6 +70,65 @@ void rf_iwt(Real *block, int wtype)
     delete f;
 }
 
+// The reference writer's block loop over a whole HDF5-ordered field
+// [nx][ny][nz] (SerializerIO_WaveletCompression_MPI_Simple.h:192-240: an
+// OpenMP loop over blocks, each gathered into uncompressed_data() and
+// compressed), blocks x fastest.  One compressor object per thread; payloads
+// are not kept, only their sizes.  Returns the payload bytes in all.
+long rf_field_compress(const Real *field, int nx, int ny, int nz, float threshold, int wtype, int nthreads,
+                       long *sizes)
+{
+    const int bx = nx / 32, by = ny / 32, bz = nz / 32, nb = bx * by * bz;
+    long total = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : total)
+    {
+        WaveletCompressor *c = new WaveletCompressor;
+#pragma omp for schedule(static)
+        for (int b = 0; b < nb; ++b) {
+            const int ix = b % bx, iy = (b / bx) % by, iz = b / (bx * by);
+            Real *buf = &c->uncompressed_data()[0][0][0];
+            for (int z = 0; z < 32; ++z)
+                for (int y = 0; y < 32; ++y)
+                    for (int x = 0; x < 32; ++x)
+                        buf[x + 32 * (y + 32 * z)] =
+                            field[((size_t)(32 * ix + x) * ny + (32 * iy + y)) * nz + (32 * iz + z)];
+            const size_t n = c->compress(threshold, false, wtype);
+            if (sizes) sizes[b] = (long)n;
+            total += (long)n;
+        }
+        delete c;
+    }
+    return total;
+}
+
+// The reference reader's per-block decode (Reader_WaveletCompression.h:
+// 700-754: decompress, then scatter into the field) for every record
+// [int32 nbytes][payload] of a stream at offsets[b], OpenMP over blocks.
+void rf_field_decompress(const unsigned char *records, const long long *offsets, int nx, int ny, int nz, int wtype,
+                         int nthreads, Real *field)
+{
+    const int bx = nx / 32, by = ny / 32, bz = nz / 32, nb = bx * by * bz;
+#pragma omp parallel num_threads(nthreads)
+    {
+        WaveletCompressor *c = new WaveletCompressor;
+        Real(*out)[32][32] = new Real[32][32][32];
+#pragma omp for schedule(static)
+        for (int b = 0; b < nb; ++b) {
+            int n;
+            std::memcpy(&n, records + offsets[b], 4);
+            std::memcpy(c->compressed_data(), records + offsets[b] + 4, (size_t)n);
+            c->decompress(false, (size_t)n, wtype, out);
+            const int ix = b % bx, iy = (b / bx) % by, iz = b / (bx * by);
+            for (int z = 0; z < 32; ++z)
+                for (int y = 0; y < 32; ++y)
+                    for (int x = 0; x < 32; ++x)
+                        field[((size_t)(32 * ix + x) * ny + (32 * iy + y)) * nz + (32 * iz + z)] = out[z][y][x];
+        }
+        delete[] out;
+        delete c;
+    }
+}
+
 void rf_row(Real *row, int n, int wtype, int forward)
 {
     using WaveletsOnInterval::WI4;
