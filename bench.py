@@ -250,6 +250,12 @@ def f64_leg(a, torch, czw, fields):
             r["cz_file_end_to_end"] = {"file_bytes": fb, "ratio": round(field_bytes / fb, 3), "threads": threads,
                                        "GBps": round(field_bytes / tm["wall"] / 1e9, 3),
                                        "seconds": {k: round(v, 4) for k, v in tm.items()}}
+            # the same writer with the reference's LZ4 second stage ("Encoder: lz4")
+            fb4, tm4 = czfile.compress_device_field(None, field, eps, a.wtype, encoder="lz4", nthreads=threads,
+                                                    overlap=True)
+            r["cz_file_end_to_end_lz4"] = {"file_bytes": fb4, "ratio": round(field_bytes / fb4, 3),
+                                           "threads": threads, "GBps": round(field_bytes / tm4["wall"] / 1e9, 3),
+                                           "seconds": {k: round(v, 4) for k, v in tm4.items()}}
         res["eps"][f"{eps:g}"] = r
     del field, slots, records, out
     torch.cuda.empty_cache()

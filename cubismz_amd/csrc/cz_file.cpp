@@ -198,8 +198,10 @@ int encode_chunks(const Params &p, const uint8_t *records, const uint64_t *offse
     return bad ? -2 : 0;
 }
 
-int assemble_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks,
-                std::vector<uint8_t> &file)
+// The file around the finished chunks, handed piece by piece to put(data,
+// bytes) in file order; returns false if put does.
+template <class Put>
+static int emit_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks, Put &&put)
 {
     const int E = entries_per_chunk(p.prec);
     if (!subdomains_valid(p) || nblocks <= 0 || p.bpd[0] * p.bpd[1] * p.bpd[2] != nblocks) return -1;
@@ -223,41 +225,70 @@ int assemble_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks
     }
     const uint64_t header_disp = sizeof(uint64_t) + ocean.size() + ocean_bytes;
 
-    file.clear();
-    file.reserve(header_disp + header.size() + nblocks * sizeof(BlockMetadata) + luttitle.size() +
-                 nsub * sizeof(HeaderLUT));
-    auto put = [&](const void *d, size_t n) {
-        const uint8_t *q = static_cast<const uint8_t *>(d);
-        file.insert(file.end(), q, q + n);
-    };
-    put(&header_disp, sizeof(header_disp));
-    put(ocean.data(), ocean.size());
+    bool ok = put(&header_disp, sizeof(header_disp)) && put(ocean.data(), ocean.size());
     std::vector<uint64_t> lut(ncs);
-    for (int s = 0; s < nsub; ++s) {
+    for (int s = 0; ok && s < nsub; ++s) {
         uint64_t w = 0;
-        for (int c = 0; c < ncs; ++c) {
+        for (int c = 0; ok && c < ncs; ++c) {
             const auto &ch = chunks[(size_t)s * ncs + c];
             lut[c] = w;
             w += ch.size();
-            put(ch.data(), ch.size());
+            ok = put(ch.data(), ch.size());
         }
-        put(lut.data(), sizeof(uint64_t) * ncs);
+        ok = ok && put(lut.data(), sizeof(uint64_t) * ncs);
     }
-    put(header.data(), header.size());
+    ok = ok && put(header.data(), header.size());
     // per subdomain, BlockMetadata {chunk, index in chunk, global ix, iy, iz}
     // of its blocks in its own order (SerializerIO_..._MPI_Simple.h:169-177,
     // 328-332; written at rank * NBLOCKS, :431-441)
     std::vector<int> order;
     file_block_order(p, order);
+    std::vector<BlockMetadata> meta(nblocks);
     const int bx = p.bpd[0], by = p.bpd[1];
     for (int q = 0; q < nblocks; ++q) {
         const int i = q % bps, g = order[q];
-        const BlockMetadata m = {i / E, i % E, g % bx, (g / bx) % by, g / (bx * by)};
-        put(&m, sizeof(m));
+        meta[q] = BlockMetadata{i / E, i % E, g % bx, (g / bx) % by, g / (bx * by)};
     }
-    put(luttitle.data(), luttitle.size());
-    put(hls.data(), sizeof(HeaderLUT) * nsub);
-    return 0;
+    ok = ok && put(meta.data(), sizeof(BlockMetadata) * (size_t)nblocks) && put(luttitle.data(), luttitle.size()) &&
+         put(hls.data(), sizeof(HeaderLUT) * nsub);
+    return ok ? 0 : -3;
+}
+
+int assemble_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks,
+                std::vector<uint8_t> &file)
+{
+    file.clear();
+    return emit_cz(p, chunks, nblocks, [&](const void *d, size_t n) {
+        const uint8_t *q = static_cast<const uint8_t *>(d);
+        file.insert(file.end(), q, q + n);
+        return true;
+    });
+}
+
+int write_cz(const std::string &path, const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks,
+             uint64_t *bytes)
+{
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return -3;
+    uint64_t n = 0;
+    int rc = emit_cz(p, chunks, nblocks, [&](const void *d, size_t k) {
+        n += k;
+        return std::fwrite(d, 1, k, f) == k;
+    });
+    if (std::fclose(f) != 0 && rc == 0) rc = -3;
+    if (bytes) *bytes = n;
+    return rc;
+}
+
+int size_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks, uint64_t *bytes)
+{
+    uint64_t n = 0;
+    const int rc = emit_cz(p, chunks, nblocks, [&](const void *, size_t k) {
+        n += k;
+        return true;
+    });
+    *bytes = n;
+    return rc;
 }
 
 int build_cz(const Params &p, const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads,

@@ -105,6 +105,11 @@ int encode_chunks(const Params &p, const uint8_t *records, const uint64_t *offse
                   std::vector<std::vector<uint8_t>> &chunks, const int *order = nullptr, int per_subdomain = 0);
 int assemble_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks,
                 std::vector<uint8_t> &file);
+// The same file written straight to path (no image in memory; -3 on an I/O
+// error), or only its size.
+int write_cz(const std::string &path, const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks,
+             uint64_t *bytes);
+int size_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks, uint64_t *bytes);
 
 struct FileInfo {
     Params p;

@@ -1,12 +1,12 @@
 // cz_pipeline.cpp -- a device field straight to a .cz file, with the host
-// zlib stage overlapped with the device compressor.
+// second stage (zlib, LZ4 or none) overlapped with the device compressor.
 //
 // The reference writer compresses a block, appends its record to the
 // thread's chunk buffer and deflates the buffer every ENTRIES blocks
 // (SerializerIO_WaveletCompression_MPI_Simple.h:187-345, 101-184).  Here the
 // field is cut into slabs of whole chunks (a multiple of ENTRIES blocks).  The
 // device compresses slab k+1 (czw_compress_field_range) while host threads
-// deflate the chunks of slab k.  The chunks, and so the file, are the same as
+// encode the chunks of slab k.  The chunks, and so the file, are the same as
 // those of one writer compressing the whole field in block order.
 #include <hip/hip_runtime_api.h>
 
@@ -122,7 +122,7 @@ extern "C" int czh_compress_device_field(const char *path, const void *field, in
         t_copy += seconds_since(tc);
         return CZH_OK;
     };
-    // Host side of slab k: the zlib stage of its chunks.
+    // Host side of slab k: the second stage of its chunks.
     auto encode = [&](int k) -> int {
         const auto th = std::chrono::steady_clock::now();
         const int b0 = k * slab, n = std::min(slab, nb - b0);
@@ -182,10 +182,10 @@ extern "C" int czh_compress_device_field(const char *path, const void *field, in
     if (rc != CZH_OK) return rc;
 
     const auto tw = std::chrono::steady_clock::now();
-    std::vector<uint8_t> file;
-    if (cz::assemble_cz(p, chunks, nb, file)) return CZH_EINVAL;
-    if (path && !cz::write_file(path, file)) return CZH_EIO;
-    if (file_bytes) *file_bytes = file.size();
+    uint64_t fsize = 0;
+    const int wrc = path ? cz::write_cz(path, p, chunks, nb, &fsize) : cz::size_cz(p, chunks, nb, &fsize);
+    if (wrc) return wrc == -3 ? CZH_EIO : CZH_EINVAL;
+    if (file_bytes) *file_bytes = fsize;
     if (times) {
         times[0] = seconds_since(t0);
         times[1] = t_gpu;

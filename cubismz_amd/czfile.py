@@ -74,10 +74,11 @@ def write(path, records, offsets, bpd, prec, wtype, threshold, encoder="zlib", n
 
 def compress_device_field(path, field, eps, wtype=3, zerobits=0, shuffle=False, encoder="zlib", nthreads=0,
                           slab_blocks=0, overlap=True):
-    """Device field (torch, HDF5 order [nx][ny][nz]) -> .cz file, the host zlib
-    stage of one slab of chunks overlapped with the device compressor on the
-    next.  path None: only the file size is computed.  Returns
-    (file_bytes, {wall, device, zlib, copy, assemble} seconds)."""
+    """Device field (torch, HDF5 order [nx][ny][nz]) -> .cz file, the host
+    second stage (encoder: zlib, lz4 or none) of one slab of chunks overlapped
+    with the device compressor on the next.  path None: only the file size is
+    computed.  Returns (file_bytes, {wall, device, encode, copy, write}
+    seconds)."""
     import torch
 
     if field.dim() != 3 or not field.is_cuda or not field.is_contiguous():
@@ -94,7 +95,7 @@ def compress_device_field(path, field, eps, wtype=3, zerobits=0, shuffle=False, 
         C.c_void_p(torch.cuda.current_stream().cuda_stream), C.byref(nbytes), times)
     if rc:
         raise CzFileError(f"czh_compress_device_field failed ({rc})")
-    keys = ("wall", "device", "zlib", "copy", "assemble")
+    keys = ("wall", "device", "encode", "copy", "write")
     return int(nbytes.value), {k: float(v) for k, v in zip(keys, times)}
 
 

@@ -77,12 +77,13 @@ int czh_lz4_decompress(const uint8_t *in, uint64_t n, uint8_t *out, uint64_t cap
  * straight to a single-writer .cz file (path may be NULL: only the size is
  * reported).  The field is cut into slabs of whole chunks of about
  * slab_blocks blocks (<= 0: 1024).  With overlap != 0 the device compresses
- * slab k+1 (czw_compress_field_range) while nthreads host threads deflate
- * slab k; with overlap == 0 the two alternate.  The bytes equal
- * czh_write of the whole field's records.  times (may be NULL) receives 5
- * values in seconds: wall, device compress, host zlib, device-to-host
- * copies, file assembly and write.  Allocates its own device and pinned
- * host buffers. */
+ * slab k+1 (czw_compress_field_range) while nthreads host threads run the
+ * second stage (encoder: zlib, LZ4 or none) on slab k; with overlap == 0 the
+ * two alternate.  The bytes equal czh_write of the whole field's records.
+ * times (may be NULL) receives 5 values in seconds: wall, device compress,
+ * host second stage, device-to-host copies, and writing the file (streamed
+ * chunk by chunk; with a NULL path only its size is summed).  Allocates its
+ * own device and pinned host buffers. */
 int czh_compress_device_field(const char *path, const void *field, int nx, int ny, int nz, int prec, float eps,
                               int wtype, int zerobits, int shuffle, int encoder, int nthreads, int slab_blocks,
                               int overlap, hipStream_t stream, uint64_t *file_bytes, double *times);
