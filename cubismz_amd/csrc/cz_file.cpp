@@ -167,8 +167,8 @@ int encode_chunks(const Params &p, const uint8_t *records, const uint64_t *offse
     chunks.assign(nchunks, std::vector<uint8_t>());
     int bad = 0;
 #ifdef _OPENMP
-    if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad) num_threads(nt)
 #endif
     for (int c = 0; c < nchunks; ++c) {
         // file positions [q0, q1) of this chunk's blocks
@@ -194,63 +194,98 @@ int encode_chunks(const Params &p, const uint8_t *records, const uint64_t *offse
             chunks[c].assign(src, src + len);
         }
     }
+#ifndef _OPENMP
     (void)nthreads;
+#endif
     return bad ? -2 : 0;
 }
 
-// The file around the finished chunks, handed piece by piece to put(data,
-// bytes) in file order; returns false if put does.
-template <class Put>
-static int emit_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks, Put &&put)
+bool CzSink::put(const void *d, size_t n)
 {
-    const int E = entries_per_chunk(p.prec);
-    if (!subdomains_valid(p) || nblocks <= 0 || p.bpd[0] * p.bpd[1] * p.bpd[2] != nblocks) return -1;
+    bytes_ += n;
+    if (mem_) {
+        const uint8_t *q = static_cast<const uint8_t *>(d);
+        mem_->insert(mem_->end(), q, q + n);
+    } else if (file_) {
+        return std::fwrite(d, 1, n, file_) == n;
+    }
+    return true;
+}
+
+bool CzSink::patch_head(uint64_t v)
+{
+    if (mem_) {
+        if (mem_->size() < sizeof(v)) return false;
+        std::memcpy(mem_->data(), &v, sizeof(v));
+    } else if (file_) {
+        return std::fseek(file_, 0, SEEK_SET) == 0 && std::fwrite(&v, 1, sizeof(v), file_) == sizeof(v) &&
+               std::fseek(file_, 0, SEEK_END) == 0;
+    }
+    return true;
+}
+
+// The layout: uint64 header displacement, the ocean title; per subdomain its
+// chunks, then uint64 lut[ncs] of chunk starts relative to its first byte,
+// so that HeaderLUT = {bytes incl. the lut, ncs}
+// (SerializerIO_..._MPI_Simple.h:563-575), subdomains in rank order
+// (MPI_Exscan of the byte counts, :385-401); then the ASCII header, the block
+// metadata, the LUT title and the HeaderLUTs.  The displacement is only known
+// at the end, so a zero goes first and finish() patches it.
+CzWriter::CzWriter(const Params &p, int nblocks, CzSink &sink) : p_(p), nblocks_(nblocks), sink_(sink)
+{
+    E_ = entries_per_chunk(p.prec);
+    ok_ = subdomains_valid(p) && nblocks > 0 && p.bpd[0] * p.bpd[1] * p.bpd[2] == nblocks;
+    if (!ok_) return;
     const int bps = p.sub_bpd[0] * p.sub_bpd[1] * p.sub_bpd[2];
-    const int nsub = nblocks / bps, ncs = (bps + E - 1) / E;
-    if ((int)chunks.size() != nsub * ncs) return -1;
-    const std::string ocean(OCEAN_TITLE), luttitle(LUT_TITLE);
-    const std::string header = make_header(p);
+    nsub_ = nblocks / bps;
+    ncs_ = (bps + E_ - 1) / E_;
+    lut_.resize(ncs_);
+}
 
-    // each subdomain writes its chunks, then uint64 lut[ncs] of chunk starts
-    // relative to its first byte; HeaderLUT {bytes incl. the lut, ncs}
-    // (SerializerIO_..._MPI_Simple.h:563-575); subdomains follow each other
-    // in rank order (MPI_Exscan of the byte counts, :385-401)
-    std::vector<HeaderLUT> hls(nsub);
-    size_t ocean_bytes = 0;
-    for (int s = 0; s < nsub; ++s) {
-        size_t w = 0;
-        for (int c = 0; c < ncs; ++c) w += chunks[(size_t)s * ncs + c].size();
-        hls[s] = HeaderLUT{(uint64_t)(w + sizeof(uint64_t) * ncs), ncs};
-        ocean_bytes += hls[s].aggregate_bytes;
-    }
-    const uint64_t header_disp = sizeof(uint64_t) + ocean.size() + ocean_bytes;
+int CzWriter::begin()
+{
+    if (!ok_ || started_) return -1;
+    started_ = true;
+    const uint64_t zero = 0;
+    return sink_.put(&zero, sizeof(zero)) && sink_.put(OCEAN_TITLE, std::strlen(OCEAN_TITLE)) ? 0 : -3;
+}
 
-    bool ok = put(&header_disp, sizeof(header_disp)) && put(ocean.data(), ocean.size());
-    std::vector<uint64_t> lut(ncs);
-    for (int s = 0; ok && s < nsub; ++s) {
-        uint64_t w = 0;
-        for (int c = 0; ok && c < ncs; ++c) {
-            const auto &ch = chunks[(size_t)s * ncs + c];
-            lut[c] = w;
-            w += ch.size();
-            ok = put(ch.data(), ch.size());
-        }
-        ok = ok && put(lut.data(), sizeof(uint64_t) * ncs);
+int CzWriter::chunk(const uint8_t *d, size_t n)
+{
+    if (!ok_ || !started_ || (int)hls_.size() == nsub_) return -1;
+    lut_[nc_] = sub_bytes_;
+    sub_bytes_ += n;
+    if (!sink_.put(d, n)) return -3;
+    if (++nc_ == ncs_) {  // the subdomain's last chunk: its lut follows
+        if (!sink_.put(lut_.data(), sizeof(uint64_t) * ncs_)) return -3;
+        hls_.push_back(HeaderLUT{sub_bytes_ + sizeof(uint64_t) * ncs_, ncs_});
+        ocean_bytes_ += hls_.back().aggregate_bytes;
+        nc_ = 0;
+        sub_bytes_ = 0;
     }
-    ok = ok && put(header.data(), header.size());
+    return 0;
+}
+
+int CzWriter::finish()
+{
+    if (!ok_ || !started_ || (int)hls_.size() != nsub_) return -1;
+    const std::string header = make_header(p_);
     // per subdomain, BlockMetadata {chunk, index in chunk, global ix, iy, iz}
     // of its blocks in its own order (SerializerIO_..._MPI_Simple.h:169-177,
     // 328-332; written at rank * NBLOCKS, :431-441)
     std::vector<int> order;
-    file_block_order(p, order);
-    std::vector<BlockMetadata> meta(nblocks);
-    const int bx = p.bpd[0], by = p.bpd[1];
-    for (int q = 0; q < nblocks; ++q) {
+    file_block_order(p_, order);
+    const int bps = nblocks_ / nsub_, bx = p_.bpd[0], by = p_.bpd[1];
+    std::vector<BlockMetadata> meta(nblocks_);
+    for (int q = 0; q < nblocks_; ++q) {
         const int i = q % bps, g = order[q];
-        meta[q] = BlockMetadata{i / E, i % E, g % bx, (g / bx) % by, g / (bx * by)};
+        meta[q] = BlockMetadata{i / E_, i % E_, g % bx, (g / bx) % by, g / (bx * by)};
     }
-    ok = ok && put(meta.data(), sizeof(BlockMetadata) * (size_t)nblocks) && put(luttitle.data(), luttitle.size()) &&
-         put(hls.data(), sizeof(HeaderLUT) * nsub);
+    const uint64_t header_disp = sizeof(uint64_t) + std::strlen(OCEAN_TITLE) + ocean_bytes_;
+    const bool ok = sink_.put(header.data(), header.size()) &&
+                    sink_.put(meta.data(), sizeof(BlockMetadata) * (size_t)nblocks_) &&
+                    sink_.put(LUT_TITLE, std::strlen(LUT_TITLE)) &&
+                    sink_.put(hls_.data(), sizeof(HeaderLUT) * (size_t)nsub_) && sink_.patch_head(header_disp);
     return ok ? 0 : -3;
 }
 
@@ -258,37 +293,12 @@ int assemble_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks
                 std::vector<uint8_t> &file)
 {
     file.clear();
-    return emit_cz(p, chunks, nblocks, [&](const void *d, size_t n) {
-        const uint8_t *q = static_cast<const uint8_t *>(d);
-        file.insert(file.end(), q, q + n);
-        return true;
-    });
-}
-
-int write_cz(const std::string &path, const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks,
-             uint64_t *bytes)
-{
-    FILE *f = std::fopen(path.c_str(), "wb");
-    if (!f) return -3;
-    uint64_t n = 0;
-    int rc = emit_cz(p, chunks, nblocks, [&](const void *d, size_t k) {
-        n += k;
-        return std::fwrite(d, 1, k, f) == k;
-    });
-    if (std::fclose(f) != 0 && rc == 0) rc = -3;
-    if (bytes) *bytes = n;
-    return rc;
-}
-
-int size_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks, uint64_t *bytes)
-{
-    uint64_t n = 0;
-    const int rc = emit_cz(p, chunks, nblocks, [&](const void *, size_t k) {
-        n += k;
-        return true;
-    });
-    *bytes = n;
-    return rc;
+    CzSink sink(&file);
+    CzWriter w(p, nblocks, sink);
+    if (!w.valid() || (int)chunks.size() != w.nchunks()) return -1;
+    int rc = w.begin();
+    for (size_t c = 0; rc == 0 && c < chunks.size(); ++c) rc = w.chunk(chunks[c].data(), chunks[c].size());
+    return rc ? rc : w.finish();
 }
 
 int build_cz(const Params &p, const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads,
@@ -430,8 +440,8 @@ int gather_records(const std::vector<uint8_t> &file, const FileInfo &info, int n
     std::vector<std::vector<uint8_t>> raw(nc);
     int bad = 0;
 #ifdef _OPENMP
-    if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad) num_threads(nt)
 #endif
     for (long long c = 0; c < (long long)nc; ++c) {
         const uint8_t *src = file.data() + info.chunk_start[c];
@@ -443,7 +453,9 @@ int gather_records(const std::vector<uint8_t> &file, const FileInfo &info, int n
             raw[c].assign(src, src + info.chunk_size[c]);
         }
     }
+#ifndef _OPENMP
     (void)nthreads;
+#endif
     if (bad) {
         if (msg) *msg = "a chunk failed to inflate";
         return -2;

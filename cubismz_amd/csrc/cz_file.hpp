@@ -18,6 +18,7 @@
 #pragma once
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -105,11 +106,47 @@ int encode_chunks(const Params &p, const uint8_t *records, const uint64_t *offse
                   std::vector<std::vector<uint8_t>> &chunks, const int *order = nullptr, int per_subdomain = 0);
 int assemble_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks,
                 std::vector<uint8_t> &file);
-// The same file written straight to path (no image in memory; -3 on an I/O
-// error), or only its size.
-int write_cz(const std::string &path, const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks,
-             uint64_t *bytes);
-int size_cz(const Params &p, const std::vector<std::vector<uint8_t>> &chunks, int nblocks, uint64_t *bytes);
+
+// Where a CzWriter puts the file: a memory image, an open FILE (binary,
+// positioned at its start), or nowhere (only the size is counted).
+class CzSink {
+  public:
+    CzSink() = default;
+    explicit CzSink(std::vector<uint8_t> *mem) : mem_(mem) {}
+    explicit CzSink(FILE *f) : file_(f) {}
+    bool put(const void *d, size_t n);
+    bool patch_head(uint64_t v);  // bytes [0, 8)
+    uint64_t bytes() const { return bytes_; }
+
+  private:
+    std::vector<uint8_t> *mem_ = nullptr;
+    FILE *file_ = nullptr;
+    uint64_t bytes_ = 0;
+};
+
+// A .cz file written front to back as its chunks become available, so that
+// only the chunks not yet written need to be held: begin(), then chunk() for
+// every chunk in file order (subdomain by subdomain, as encode_chunks cuts
+// them), then finish().  Each returns 0, -1 on misuse or bad parameters, or
+// -3 when the sink fails.
+class CzWriter {
+  public:
+    CzWriter(const Params &p, int nblocks, CzSink &sink);
+    bool valid() const { return ok_; }
+    int nchunks() const { return ok_ ? nsub_ * ncs_ : 0; }
+    int begin();
+    int chunk(const uint8_t *d, size_t n);
+    int finish();
+
+  private:
+    Params p_;
+    int nblocks_, E_ = 1, nsub_ = 0, ncs_ = 0, nc_ = 0;
+    bool ok_ = false, started_ = false;
+    CzSink &sink_;
+    std::vector<uint64_t> lut_;
+    std::vector<HeaderLUT> hls_;
+    uint64_t sub_bytes_ = 0, ocean_bytes_ = 0;
+};
 
 struct FileInfo {
     Params p;

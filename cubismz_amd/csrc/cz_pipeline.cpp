@@ -6,12 +6,15 @@
 // (SerializerIO_WaveletCompression_MPI_Simple.h:187-345, 101-184).  Here the
 // field is cut into slabs of whole chunks (a multiple of ENTRIES blocks).  The
 // device compresses slab k+1 (czw_compress_field_range) while host threads
-// encode the chunks of slab k.  The chunks, and so the file, are the same as
+// encode the chunks of slab k, and each slab's chunks go to the file as soon
+// as they are encoded (cz::CzWriter), so host memory holds two slabs of
+// records and one of chunks.  The chunks, and so the file, are the same as
 // those of one writer compressing the whole field in block order.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -34,6 +37,32 @@ struct DeviceSlab {
         if (records) (void)hipFree(records);
         if (workspace) (void)hipFree(workspace);
         if (offsets) (void)hipFree(offsets);
+    }
+};
+
+struct Events {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    bool create()
+    {
+        for (auto &x : e)
+            if (hipEventCreate(&x) != hipSuccess) return false;
+        return true;
+    }
+    ~Events()
+    {
+        for (auto x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
+struct OutFile {
+    FILE *f = nullptr;
+    const char *path = nullptr;
+    bool keep = false;
+    ~OutFile()
+    {
+        if (f) std::fclose(f);
+        if (path && !keep) std::remove(path);  // no partial file is left behind
     }
 };
 
@@ -83,10 +112,9 @@ extern "C" int czh_compress_device_field(const char *path, const void *field, in
     HostBuf rec_h[2], off_h[2];
     for (auto &o : off_h)
         if (!o.reserve(sizeof(uint64_t) * (slab + 1))) return CZH_EIO;
-    hipEvent_t ev_c0, ev_c1, ev_off;
-    if (hipEventCreate(&ev_c0) != hipSuccess || hipEventCreate(&ev_c1) != hipSuccess ||
-        hipEventCreate(&ev_off) != hipSuccess)
-        return CZH_EIO;
+    Events ev;
+    if (!ev.create()) return CZH_EIO;
+    hipEvent_t ev_c0 = ev.e[0], ev_c1 = ev.e[1], ev_off = ev.e[2];
 
     cz::Params p;
     p.prec = prec;
@@ -94,10 +122,22 @@ extern "C" int czh_compress_device_field(const char *path, const void *field, in
     p.wtype = wtype;
     p.threshold = eps;
     p.encoder = encoder;
-    std::vector<std::vector<uint8_t>> chunks;
-    chunks.reserve((nb + E - 1) / E);
-    double t_gpu = 0, t_host = 0, t_copy = 0;
+    OutFile out;
+    if (path) {
+        out.f = std::fopen(path, "wb");
+        if (!out.f) return CZH_EIO;
+        out.path = path;
+    }
+    cz::CzSink sink = path ? cz::CzSink(out.f) : cz::CzSink();
+    cz::CzWriter writer(p, nb, sink);
+    if (!writer.valid()) return CZH_EINVAL;
+    double t_gpu = 0, t_host = 0, t_copy = 0, t_write = 0;
     int rc = CZH_OK;
+    {
+        const auto tw = std::chrono::steady_clock::now();
+        if (writer.begin()) return CZH_EIO;
+        t_write += seconds_since(tw);
+    }
 
     // Device side of slab k: compress, then copy its offsets and records to
     // host buffer k % 2.  Returns with the copies queued on the stream.
@@ -122,16 +162,21 @@ extern "C" int czh_compress_device_field(const char *path, const void *field, in
         t_copy += seconds_since(tc);
         return CZH_OK;
     };
-    // Host side of slab k: the second stage of its chunks.
+    // Host side of slab k: the second stage of its chunks, which then go to
+    // the file.
     auto encode = [&](int k) -> int {
         const auto th = std::chrono::steady_clock::now();
         const int b0 = k * slab, n = std::min(slab, nb - b0);
         std::vector<std::vector<uint8_t>> part;
         const int e = cz::encode_chunks(p, rec_h[k & 1].p, reinterpret_cast<const uint64_t *>(off_h[k & 1].p), n,
                                         nthreads, part);
-        for (auto &c : part) chunks.push_back(std::move(c));
         t_host += seconds_since(th);
-        return e ? CZH_EZLIB : CZH_OK;
+        if (e) return CZH_EZLIB;
+        const auto tw = std::chrono::steady_clock::now();
+        for (const auto &c : part)
+            if (writer.chunk(c.data(), c.size())) return CZH_EIO;
+        t_write += seconds_since(tw);
+        return CZH_OK;
     };
 
     rc = issue(0);
@@ -176,22 +221,24 @@ extern "C" int czh_compress_device_field(const char *path, const void *field, in
             if (rc == CZH_OK && k + 1 < nslabs) rc = issue(k + 1);
         }
     }
-    (void)hipEventDestroy(ev_c0);
-    (void)hipEventDestroy(ev_c1);
-    (void)hipEventDestroy(ev_off);
     if (rc != CZH_OK) return rc;
 
     const auto tw = std::chrono::steady_clock::now();
-    uint64_t fsize = 0;
-    const int wrc = path ? cz::write_cz(path, p, chunks, nb, &fsize) : cz::size_cz(p, chunks, nb, &fsize);
-    if (wrc) return wrc == -3 ? CZH_EIO : CZH_EINVAL;
-    if (file_bytes) *file_bytes = fsize;
+    if (writer.finish()) return CZH_EIO;
+    if (out.f) {
+        const int c = std::fclose(out.f);
+        out.f = nullptr;
+        if (c != 0) return CZH_EIO;
+    }
+    out.keep = true;
+    t_write += seconds_since(tw);
+    if (file_bytes) *file_bytes = sink.bytes();
     if (times) {
         times[0] = seconds_since(t0);
         times[1] = t_gpu;
         times[2] = t_host;
         times[3] = t_copy;
-        times[4] = seconds_since(tw);
+        times[4] = t_write;
     }
     return CZH_OK;
 }

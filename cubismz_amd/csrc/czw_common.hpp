@@ -330,12 +330,9 @@ inline size_t size_words(int prec) { return prec == 8 ? 9 : 2; }
 // Half-block kernels (czw_half.hip) for Real of prec bytes; layout is
 // LAYOUT_BLOCKS, LAYOUT_FIELD(_WIDE) or LAYOUT_BATCH (then src/dst come from
 // the field table); groups = 0 launches every resident workgroup; queue =
-// nullptr keeps the static block order.  The *_f64 forms are prec 8.
-int compress_slots_half(int prec, int layout, const void *src, int nblocks, FieldGeom g, float eps, int wtype,
-                        int zerobits, uint8_t *slots, uint32_t *nbytes, const czw_field_t *fields, int nf, int groups,
-                        uint32_t *queue, hipStream_t s);
-int pack_records_half(int prec, const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle,
-                      uint8_t *records, uint64_t *offsets, hipStream_t s);
+// nullptr keeps the static block order.  The compressor and packer here are
+// binary64 only (binary32 compresses in czw_kernels.hip); the decoder takes
+// both precisions.
 int decompress_half(int prec, int layout, const void *records, size_t rec_bytes, const uint64_t *offsets,
                     int nblocks, FieldGeom g, int wtype, int shuffle, void *dst, uint32_t *err,
                     const czw_field_t *fields, int nf, int groups, uint32_t *queue, hipStream_t s);
@@ -344,8 +341,5 @@ int compress_slots_f64(int layout, const void *src, int nblocks, FieldGeom g, fl
                        uint32_t *queue, hipStream_t s);
 int pack_records_f64(const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle, uint8_t *records,
                      uint64_t *offsets, hipStream_t s);
-int decompress_f64(int layout, const void *records, size_t rec_bytes, const uint64_t *offsets, int nblocks,
-                   FieldGeom g, int wtype, int shuffle, void *dst, uint32_t *err, const czw_field_t *fields, int nf,
-                   int groups, uint32_t *queue, hipStream_t s);
 
 }  // namespace czw

@@ -1,8 +1,9 @@
 // czw_half.hip -- block compressor and decompressor that keep one x-half of a
-// 32^3 block in LDS at a time (WaveletCompressor.cpp:153-259), for Real =
-// double (the reference's double build: a 256 KB block does not fit in the
-// 160 KB of LDS) and Real = float (a 64 KB half lets two workgroups share a
-// CU, so one's memory phases overlap the other's arithmetic):
+// 32^3 block in LDS at a time (WaveletCompressor.cpp:153-259).  The
+// compressor serves Real = double (the reference's double build: a 256 KB
+// block does not fit in the 160 KB of LDS); the decompressor serves double
+// and float (a 64 KB half lets two workgroups share a CU, so one's memory
+// phases overlap the other's arithmetic):
 //
 //   compress:   level-32 x-pass on the thread's two x-rows in registers;
 //               x >= 16 half -> LDS: y-pass, z-pass, threshold, survivors
@@ -708,34 +709,27 @@ static int compress_slots_t(int layout, const void *src, int nblocks, FieldGeom 
     }
 }
 
-int compress_slots_half(int prec, int layout, const void *src, int nblocks, FieldGeom g, float eps, int wtype,
-                        int zerobits, uint8_t *slots, uint32_t *nbytes, const czw_field_t *fields, int nf, int groups,
-                        uint32_t *queue, hipStream_t s)
+// The binary32 compressor is k_compress_f32 (czw_kernels.hip); the half-block
+// compressor and its packer are instantiated for binary64 only.
+int compress_slots_f64(int layout, const void *src, int nblocks, FieldGeom g, float eps, int wtype, int zerobits,
+                       uint8_t *slots, uint32_t *nbytes, const czw_field_t *fields, int nf, int groups,
+                       uint32_t *queue, hipStream_t s)
 {
     const int wt = wt_index(wtype);
     if (wt < 0 || !zb_ok(zerobits) || nblocks <= 0 || !slots || !nbytes) return CZW_EINVAL;
     if (layout != LAYOUT_BATCH && !src) return CZW_EINVAL;
     const uint32_t zkeep = zerobits ? ~((1u << zerobits) - 1u) : 0xffffffffu;
-    if (prec == 8)
-        return compress_slots_t<double>(layout, src, nblocks, g, eps, wt, zkeep, slots, nbytes, fields, nf, groups,
-                                        queue, s);
-    if (prec == 4)
-        return compress_slots_t<float>(layout, src, nblocks, g, eps, wt, zkeep, slots, nbytes, fields, nf, groups,
-                                       queue, s);
-    return CZW_EINVAL;
+    return compress_slots_t<double>(layout, src, nblocks, g, eps, wt, zkeep, slots, nbytes, fields, nf, groups,
+                                    queue, s);
 }
 
-int pack_records_half(int prec, const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle,
-                      uint8_t *records, uint64_t *offsets, hipStream_t s)
+int pack_records_f64(const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle, uint8_t *records,
+                     uint64_t *offsets, hipStream_t s)
 {
-    if (nblocks <= 0 || !slots || !nbytes || !records || !offsets || (prec != 4 && prec != 8)) return CZW_EINVAL;
+    if (nblocks <= 0 || !slots || !nbytes || !records || !offsets) return CZW_EINVAL;
     if (scan_records(nbytes, nblocks, offsets, s) != CZW_OK) return CZW_EHIP;
-    if (prec == 8)
-        hipLaunchKernelGGL(k_pack_records_h<double>, dim3(nblocks), dim3(256), 0, s, slots, nbytes, nbytes + nblocks,
-                           shuffle, offsets, records);
-    else
-        hipLaunchKernelGGL(k_pack_records_h<float>, dim3(nblocks), dim3(256), 0, s, slots, nbytes, nbytes + nblocks,
-                           shuffle, offsets, records);
+    hipLaunchKernelGGL(k_pack_records_h<double>, dim3(nblocks), dim3(256), 0, s, slots, nbytes, nbytes + nblocks,
+                       shuffle, offsets, records);
     return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
 }
 
@@ -803,28 +797,6 @@ int decompress_half(int prec, int layout, const void *records, size_t rec_bytes,
         return decompress_t<float>(layout, rec, (uint64_t)rec_bytes, offsets, nblocks, g, wt, shuffle, dst, err,
                                    fields, nf, groups, queue, s);
     return CZW_EINVAL;
-}
-
-int compress_slots_f64(int layout, const void *src, int nblocks, FieldGeom g, float eps, int wtype, int zerobits,
-                       uint8_t *slots, uint32_t *nbytes, const czw_field_t *fields, int nf, int groups,
-                       uint32_t *queue, hipStream_t s)
-{
-    return compress_slots_half(8, layout, src, nblocks, g, eps, wtype, zerobits, slots, nbytes, fields, nf, groups,
-                               queue, s);
-}
-
-int pack_records_f64(const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle, uint8_t *records,
-                     uint64_t *offsets, hipStream_t s)
-{
-    return pack_records_half(8, slots, nbytes, nblocks, shuffle, records, offsets, s);
-}
-
-int decompress_f64(int layout, const void *records, size_t rec_bytes, const uint64_t *offsets, int nblocks,
-                   FieldGeom g, int wtype, int shuffle, void *dst, uint32_t *err, const czw_field_t *fields, int nf,
-                   int groups, uint32_t *queue, hipStream_t s)
-{
-    return decompress_half(8, layout, records, rec_bytes, offsets, nblocks, g, wtype, shuffle, dst, err, fields, nf,
-                           groups, queue, s);
 }
 
 }  // namespace czw
