@@ -494,6 +494,10 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
     load_table<LAYOUT>(T, fields, nf);
     const int xh = t & 15, sl = t >> 4;
     int bn = 0;
+    // The record offset of the next block is read one iteration ahead, and a
+    // block's size word and mask are read together, so that a block starts
+    // after one memory round trip instead of three dependent ones.
+    uint64_t obn = (int)blockIdx.x < nblocks ? offsets[blockIdx.x] : 0;
     for (int b = blockIdx.x, it = 0; b < nblocks; ++it, b = bn) {
         if (queue) {
             if (t == 0) ticket[it & 1] = G + (int)atomicAdd(queue, 1u);
@@ -502,23 +506,30 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
         } else {
             bn = b + G;
         }
-        const uint64_t ob = offsets[b];
+        const uint64_t ob = obn;
+        if (bn < nblocks) obn = offsets[bn];
         if ((ob & 3u) || ob + 4 + MASK_BYTES > rec_bytes) {
             if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_NBYTES);
             continue;
         }
+        // the mask lies inside the record (checked above); it is stored
+        // before the size word is checked, so that both loads are in flight
+        // together (a bad size leaves M to be overwritten by the next block)
         const uint32_t *p = reinterpret_cast<const uint32_t *>(records + ob);
+        uint32_t mk[NROW];
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) mk[k] = p[1 + t + NT * k];
         const uint32_t nb = p[0];
+        __syncthreads();  // the previous block's last reads of H, M, cnt and flag are done
+#pragma unroll
+        for (int k = 0; k < NROW; ++k) M[t + NT * k] = mk[k];
         if (nb < MASK_BYTES || ((nb - MASK_BYTES) % sizeof(R)) || nb > slot_of<R>() || ob + 4 + nb > rec_bytes) {
             if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_NBYTES);
             continue;
         }
         const uint32_t S = (nb - MASK_BYTES) / (uint32_t)sizeof(R);
         const Payload<R, SHUF> pay{p + 1 + MASK_BYTES / 4, records + ob + 4 + MASK_BYTES, S};
-        __syncthreads();  // the previous block's last reads of H, M, cnt and flag are done
         if (t == 0) flag = 1u;
-#pragma unroll
-        for (int k = 0; k < NROW; ++k) M[t + NT * k] = p[1 + t + NT * k];
         __syncthreads();
         // the 4^3 base is always stored (FullWaveletTransform.h:163-176)
         if (t < 16 && (M[(t & 3) + 32 * (t >> 2)] & 0xfu) != 0xfu) flag = 0u;

@@ -103,6 +103,12 @@ def test_corrupt_record_sets_error(torch_gpu):
     bad[o + 4 + 100] ^= 0xFF  # flip mask bits: popcount no longer matches
     with pytest.raises(czw.CZWError):
         czw.decompress_field(bad, off, SHAPE, torch.float32, 3)
+    for b, shift in ((1, 1), (2, rec.numel())):  # a misaligned record, one past the end
+        badoff = off.clone()
+        badoff[b] += shift
+        with pytest.raises(czw.CZWError):
+            czw.decompress_field(rec, badoff, SHAPE, torch.float32, 3)
+    czw.decompress_field(rec, off, SHAPE, torch.float32, 3)  # the good stream still decodes
 
 
 @pytest.mark.parametrize("wtype", [1, 2, 3])
