@@ -149,3 +149,36 @@ def test_dense_and_mixed_blocks(torch_gpu, oracle, wtype, shuffle):
         out = czw.decompress_field(rec, off, SHAPE, torch.float32, wtype, shuffle).cpu().numpy()
         ref = oracle.decompress_field(slots, nbytes, SHAPE, wtype, "f32", shuffle)
         assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("wtype", [1, 2, 3])
+def test_special_values_match_oracle(torch_gpu, oracle, prec, wtype):
+    """Blocks of subnormals, signed zeros, overflowing predictors,
+    infinities and NaNs beside ordinary ones: the same masks, survivors and
+    decoded values as the oracle (NaN payloads aside; the oracle is checked
+    against the compiled reference on the same blocks)."""
+    from tests.test_oracle import nan_aware_equal, special_block
+
+    torch = torch_gpu
+    print(f"[case] special values {prec} wtype={wtype}", flush=True)
+    dt = np.float32 if prec == "f32" else np.float64
+    rng = np.random.default_rng(21)
+    f = _field(torch, seed=4, kind="bumps", dtype=torch.float32 if prec == "f32" else torch.float64).numpy().copy()
+    kinds = ["subnormal", "zeros", "huge", "inf", "nan"]
+    for b, kind in enumerate(kinds * 2):  # blocks 0..9 special, the rest ordinary
+        ix, iy, iz = b % 2, (b // 2) % 3, b // 6
+        f[32 * ix:32 * ix + 32, 32 * iy:32 * iy + 32, 32 * iz:32 * iz + 32] = special_block(dt, rng, kind)
+    for eps in (1e-3, 0.0):
+        rec, off = czw.compress_field(torch.from_numpy(f).cuda(), eps, wtype, 0, False)
+        got = _payloads(rec, off)
+        slots, nbytes = oracle.compress_field(f, eps, wtype, 0, False)
+        for b, p in enumerate(got):
+            ref = slots[b, :nbytes[b]].tobytes()
+            assert len(p) == len(ref) and p[:4096] == ref[:4096], f"eps={eps} block {b}"
+            assert nan_aware_equal(np.frombuffer(p[4096:], np.uint8), np.frombuffer(ref[4096:], np.uint8), dt), \
+                f"eps={eps} block {b}"
+        out = czw.decompress_field(rec, off, SHAPE, torch.float32 if prec == "f32" else torch.float64, wtype,
+                                   False).cpu().numpy()
+        ref = oracle.decompress_field(slots, nbytes, SHAPE, wtype, prec, False)
+        assert nan_aware_equal(out, ref, dt), f"eps={eps}"

@@ -247,3 +247,56 @@ def test_oracle_matches_compiled_reference_random(oracle, prec):
                 assert np.array_equal(p, oracle.ref_compress(blk, 1e-2, w, zb, sh))
                 assert np.array_equal(oracle.decompress(p, w, prec, sh).view(np.uint8),
                                       oracle.ref_decompress(p, w, prec, zb, sh).view(np.uint8))
+
+
+def special_block(dt, rng, kind):
+    """A smooth block with one kind of special value planted: subnormals,
+    signed zeros, values whose predictors round past the largest finite Real,
+    infinities or NaNs."""
+    i = np.arange(32, dtype=np.float64)
+    blk = 1.0 + 0.3 * np.sin(i[:, None, None] / 5.0) * np.cos(i[None, :, None] / 7.0) + 0.01 * i[None, None, :]
+    blk = blk.astype(dt)
+    fi = np.finfo(dt)
+    pos = tuple(rng.integers(0, 32, (3, 40)))
+    if kind == "subnormal":
+        blk *= fi.tiny  # the whole block below the normal range
+        blk[pos] = fi.smallest_subnormal * rng.integers(1, 50, 40)
+    elif kind == "zeros":
+        blk[:, :, :16] = 0.0
+        blk[pos] = -0.0
+    elif kind == "huge":
+        blk[pos] = fi.max * rng.choice([-0.9, 0.9], 40)
+    elif kind == "inf":
+        blk[pos] = rng.choice([-np.inf, np.inf], 40)
+    elif kind == "nan":
+        blk[pos] = np.nan
+    return blk
+
+
+def nan_aware_equal(a, b, dt):
+    """Equal bit for bit except that a NaN matches any NaN (NaN payloads are
+    not pinned across hardware)."""
+    a, b = np.asarray(a).view(dt), np.asarray(b).view(dt)
+    na, nb = np.isnan(a), np.isnan(b)
+    it = np.uint32 if dt == np.float32 else np.uint64
+    return bool(np.array_equal(na, nb) and np.array_equal(a[~na].view(it), b[~nb].view(it)))
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("kind", ["subnormal", "zeros", "huge", "inf", "nan"])
+def test_oracle_matches_compiled_reference_special_values(oracle, prec, kind):
+    """Subnormals, signed zeros, overflowing predictors, infinities and NaNs:
+    the oracle keeps the reference's masks and values (NaN payloads aside)."""
+    if not oracle.ref_available(prec, 0, False):
+        pytest.skip("oracle/_ref not built here")
+    dt = np.float32 if prec == "f32" else np.float64
+    rng = np.random.default_rng(21)
+    blk = special_block(dt, rng, kind)
+    eps = 0.0 if kind == "subnormal" else 1e-3
+    for w in (1, 2, 3):
+        p = oracle.compress(blk, eps, w)
+        r = oracle.ref_compress(blk, eps, w)
+        assert len(p) == len(r) and np.array_equal(p[:4096], r[:4096]), (kind, w)
+        assert nan_aware_equal(p[4096:], r[4096:], dt), (kind, w)
+        d = oracle.decompress(r, w, prec)
+        assert nan_aware_equal(d, oracle.ref_decompress(r, w, prec), dt), (kind, w)
