@@ -182,3 +182,22 @@ def test_special_values_match_oracle(torch_gpu, oracle, prec, wtype):
                                    False).cpu().numpy()
         ref = oracle.decompress_field(slots, nbytes, SHAPE, wtype, prec, False)
         assert nan_aware_equal(out, ref, dt), f"eps={eps}"
+
+
+@pytest.mark.parametrize("wtype", [1, 3])
+def test_extreme_thresholds(torch_gpu, oracle, wtype):
+    """eps above every coefficient keeps only the 4^3 base of each block; a
+    negative eps keeps every coefficient, zeros included."""
+    torch = torch_gpu
+    print(f"[case] extreme eps wtype={wtype}", flush=True)
+    f = _field(torch, seed=6)
+    for eps, S in ((1e30, 64), (float("inf"), 64), (-1.0, 32768)):
+        rec, off = czw.compress_field(f.cuda(), eps, wtype, 0, False)
+        got = _payloads(rec, off)
+        slots, nbytes = oracle.compress_field(f.numpy(), eps, wtype, 0, False)
+        assert all(len(p) == 4096 + 4 * S for p in got), eps
+        for b, p in enumerate(got):
+            assert p == slots[b, :nbytes[b]].tobytes(), f"eps={eps} block {b}"
+        out = czw.decompress_field(rec, off, SHAPE, torch.float32, wtype, False).cpu().numpy()
+        ref = oracle.decompress_field(slots, nbytes, SHAPE, wtype, "f32", False)
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), eps
