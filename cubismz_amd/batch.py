@@ -107,7 +107,13 @@ class Batch:
         self.fields = list(fields)
         self.table, self.nblocks = prepare_table([tuple(f.shape) for f in fields], eps, czw._prec(dtype, torch))
         dev = self.fields[0].device
+        if any(f.device != dev for f in self.fields):
+            raise czw.CZWError("batch fields must be on one device")
         self.outputs = list(outputs) if outputs is not None else [torch.empty_like(f) for f in fields]
+        if len(self.outputs) != len(self.fields):
+            raise czw.CZWError("one output per field")
+        for k, (f, o) in enumerate(zip(self.fields, self.outputs)):
+            czw._need(o, f"outputs[{k}]", torch, f.dtype, shape=f.shape, device=dev)
         for k, f in enumerate(self.fields):
             self.table[k]["src"] = f.data_ptr()
             self.table[k]["dst"] = self.outputs[k].data_ptr()

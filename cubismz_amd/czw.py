@@ -98,6 +98,33 @@ def _prec(dtype, torch):
     raise CZWError(f"unsupported dtype {dtype}")
 
 
+def _need(t, name, torch, dtype=None, min_bytes=None, shape=None, device=None):
+    """A caller's device buffer: contiguous, on the GPU (and on `device`), of
+    `dtype` (one or a tuple), exactly `shape`, at least `min_bytes` long."""
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or not t.is_contiguous():
+        raise CZWError(f"{name} must be a contiguous device tensor")
+    if device is not None and t.device != device:
+        raise CZWError(f"{name} is on {t.device}, expected {device}")
+    if dtype is not None and t.dtype not in (dtype if isinstance(dtype, tuple) else (dtype,)):
+        raise CZWError(f"{name} has dtype {t.dtype}")
+    if shape is not None and tuple(t.shape) != tuple(shape):
+        raise CZWError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if min_bytes is not None and t.numel() * t.element_size() < min_bytes:
+        raise CZWError(f"{name} too small: {t.numel() * t.element_size()} bytes, need {min_bytes}")
+    return t
+
+
+def _field_arg(field, torch):
+    if not isinstance(field, torch.Tensor) or field.dim() != 3 or not field.is_cuda or not field.is_contiguous():
+        raise CZWError("field must be a contiguous 3D device tensor")
+    return tuple(int(s) for s in field.shape)
+
+
+def size_words(prec):
+    """uint32 words per block in a compressor's size array (nbytes)."""
+    return (workspace_bytes(1, prec) - slot_bytes(prec)) // 4
+
+
 def workspace_bytes(nblocks, prec):
     return int(load_library().czw_compress_workspace_bytes(int(nblocks), int(prec)))
 
@@ -113,10 +140,9 @@ def _buffers(torch, device, nblocks, prec, records, offsets, workspace):
         offsets = torch.empty(nblocks + 1, dtype=torch.int64, device=device)
     if workspace is None:
         workspace = torch.empty(workspace_bytes(nblocks, prec), dtype=torch.uint8, device=device)
-    if records.numel() < records_bound(nblocks, prec) or offsets.numel() < nblocks + 1:
-        raise CZWError("records/offsets buffers too small")
-    if workspace.numel() < workspace_bytes(nblocks, prec):
-        raise CZWError("workspace too small")
+    _need(records, "records", torch, torch.uint8, records_bound(nblocks, prec), device=device)
+    _need(offsets, "offsets", torch, torch.int64, 8 * (nblocks + 1), device=device)
+    _need(workspace, "workspace", torch, torch.uint8, workspace_bytes(nblocks, prec), device=device)
     return records, offsets, workspace
 
 
@@ -128,9 +154,7 @@ def compress_field(field, eps, wtype=3, zerobits=0, shuffle=False, records=None,
     """
     torch = _torch()
     L = load_library()
-    if field.dim() != 3 or not field.is_cuda or not field.is_contiguous():
-        raise CZWError("field must be a contiguous 3D device tensor")
-    nx, ny, nz = (int(s) for s in field.shape)
+    nx, ny, nz = _field_arg(field, torch)
     prec = _prec(field.dtype, torch)
     nblocks = (nx // BLOCK) * (ny // BLOCK) * (nz // BLOCK)
     records, offsets, workspace = _buffers(torch, field.device, nblocks, prec, records, offsets, workspace)
@@ -144,10 +168,10 @@ def compress_field(field, eps, wtype=3, zerobits=0, shuffle=False, records=None,
 def compress_field_range(field, eps, wtype, zerobits, shuffle, block0, nblocks):
     """compress_field for the field's blocks [block0, block0 + nblocks) only."""
     torch = _torch()
-    if field.dim() != 3 or not field.is_cuda or not field.is_contiguous():
-        raise CZWError("field must be a contiguous 3D device tensor")
-    nx, ny, nz = (int(s) for s in field.shape)
+    nx, ny, nz = _field_arg(field, torch)
     prec = _prec(field.dtype, torch)
+    if int(nblocks) <= 0:
+        raise CZWError("nblocks must be positive")
     records, offsets, workspace = _buffers(torch, field.device, nblocks, prec, None, None, None)
     rc = load_library().czw_compress_field_range(C.c_void_p(field.data_ptr()), nx, ny, nz, prec, float(eps),
                                                  int(wtype), int(zerobits), int(bool(shuffle)), int(block0),
@@ -160,11 +184,16 @@ def compress_field_range(field, eps, wtype, zerobits, shuffle, block0, nblocks):
 
 def compress_field_slots(field, eps, wtype, zerobits, slots, nbytes):
     """Stage 1 of compress_field: per-block payloads into fixed-size slots
-    (nbytes: 2*nblocks uint32, see czw.h)."""
+    (slots: nblocks * slot_bytes(prec) bytes; nbytes: size_words(prec) *
+    nblocks int32, see czw.h)."""
     torch = _torch()
     L = load_library()
-    nx, ny, nz = (int(s) for s in field.shape)
-    rc = L.czw_compress_field_slots(C.c_void_p(field.data_ptr()), nx, ny, nz, _prec(field.dtype, torch), float(eps),
+    nx, ny, nz = _field_arg(field, torch)
+    prec = _prec(field.dtype, torch)
+    nb = (nx // BLOCK) * (ny // BLOCK) * (nz // BLOCK)
+    _need(slots, "slots", torch, torch.uint8, nb * slot_bytes(prec), device=field.device)
+    _need(nbytes, "nbytes", torch, torch.int32, 4 * size_words(prec) * nb, device=field.device)
+    rc = L.czw_compress_field_slots(C.c_void_p(field.data_ptr()), nx, ny, nz, prec, float(eps),
                                     int(wtype), int(zerobits), C.c_void_p(slots.data_ptr()),
                                     C.c_void_p(nbytes.data_ptr()), _stream(torch))
     _check(rc, "czw_compress_field_slots")
@@ -173,9 +202,18 @@ def compress_field_slots(field, eps, wtype, zerobits, slots, nbytes):
 def pack_records(slots, slot_bytes, nbytes, nblocks, shuffle, records, offsets):
     """Stage 2 of compress_field: frame slot payloads into the record stream."""
     torch = _torch()
-    rc = load_library().czw_pack_records(C.c_void_p(slots.data_ptr()), int(slot_bytes), C.c_void_p(nbytes.data_ptr()),
-                                         int(nblocks), int(bool(shuffle)), C.c_void_p(records.data_ptr()),
-                                         C.c_void_p(offsets.data_ptr()), _stream(torch))
+    L = load_library()
+    prec = {int(L.czw_slot_bytes(4)): 4, int(L.czw_slot_bytes(8)): 8}.get(int(slot_bytes))
+    if prec is None or int(nblocks) <= 0:
+        raise CZWError("slot_bytes must be slot_bytes(4) or slot_bytes(8), nblocks positive")
+    dev = slots.device
+    _need(slots, "slots", torch, torch.uint8, int(nblocks) * int(slot_bytes))
+    _need(nbytes, "nbytes", torch, torch.int32, 4 * size_words(prec) * int(nblocks), device=dev)
+    _need(records, "records", torch, torch.uint8, records_bound(int(nblocks), prec), device=dev)
+    _need(offsets, "offsets", torch, torch.int64, 8 * (int(nblocks) + 1), device=dev)
+    rc = L.czw_pack_records(C.c_void_p(slots.data_ptr()), int(slot_bytes), C.c_void_p(nbytes.data_ptr()),
+                            int(nblocks), int(bool(shuffle)), C.c_void_p(records.data_ptr()),
+                            C.c_void_p(offsets.data_ptr()), _stream(torch))
     _check(rc, "czw_pack_records")
 
 
@@ -190,6 +228,8 @@ def compress_blocks(blocks, eps, wtype=3, zerobits=0, shuffle=False, records=Non
     if blocks.dim() != 4 or tuple(blocks.shape[1:]) != (BLOCK,) * 3 or not blocks.is_cuda or not blocks.is_contiguous():
         raise CZWError("blocks must be a contiguous device tensor [nb][32][32][32]")
     nblocks = int(blocks.shape[0])
+    if nblocks == 0:
+        raise CZWError("no blocks")
     prec = _prec(blocks.dtype, torch)
     records, offsets, workspace = _buffers(torch, blocks.device, nblocks, prec, records, offsets, workspace)
     rc = L.czw_compress_blocks(C.c_void_p(blocks.data_ptr()), nblocks, prec, float(eps), int(wtype), int(zerobits),
@@ -202,7 +242,12 @@ def compress_blocks(blocks, eps, wtype=3, zerobits=0, shuffle=False, records=Non
 def _err_buffer(torch, device, err):
     if err is None:
         err = torch.zeros(1, dtype=torch.int32, device=device)
-    return err
+    return _need(err, "err", torch, torch.int32, 4, device=device)
+
+
+def _decode_inputs(torch, records, offsets, nblocks):
+    _need(records, "records", torch, torch.uint8)
+    _need(offsets, "offsets", torch, torch.int64, 8 * (nblocks + 1), device=records.device)
 
 
 def _raise_err(err):
@@ -217,8 +262,10 @@ def decompress_field(records, offsets, shape, dtype, wtype=3, shuffle=False, out
     L = load_library()
     nx, ny, nz = (int(s) for s in shape)
     prec = _prec(dtype, torch)
+    _decode_inputs(torch, records, offsets, (nx // BLOCK) * (ny // BLOCK) * (nz // BLOCK))
     if out is None:
         out = torch.empty((nx, ny, nz), dtype=dtype, device=records.device)
+    _need(out, "out", torch, dtype, shape=(nx, ny, nz), device=records.device)
     err = _err_buffer(torch, records.device, err)
     rc = L.czw_decompress_field(C.c_void_p(records.data_ptr()), records.numel() * records.element_size(),
                                 C.c_void_p(offsets.data_ptr()), nx, ny, nz, prec,
@@ -235,8 +282,10 @@ def decompress_blocks(records, offsets, nblocks, dtype, wtype=3, shuffle=False, 
     torch = _torch()
     L = load_library()
     prec = _prec(dtype, torch)
+    _decode_inputs(torch, records, offsets, int(nblocks))
     if out is None:
         out = torch.empty((nblocks, BLOCK, BLOCK, BLOCK), dtype=dtype, device=records.device)
+    _need(out, "out", torch, dtype, shape=(int(nblocks), BLOCK, BLOCK, BLOCK), device=records.device)
     err = _err_buffer(torch, records.device, err)
     rc = L.czw_decompress_blocks(C.c_void_p(records.data_ptr()), records.numel() * records.element_size(),
                                  C.c_void_p(offsets.data_ptr()), int(nblocks), prec,

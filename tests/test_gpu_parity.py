@@ -201,3 +201,49 @@ def test_extreme_thresholds(torch_gpu, oracle, wtype):
         out = czw.decompress_field(rec, off, SHAPE, torch.float32, wtype, False).cpu().numpy()
         ref = oracle.decompress_field(slots, nbytes, SHAPE, wtype, "f32", False)
         assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), eps
+
+
+def test_buffer_validation(torch_gpu):
+    """Caller buffers of the wrong size, dtype, shape or device are refused
+    before any launch."""
+    torch = torch_gpu
+    print("[case] buffer validation", flush=True)
+    f = _field(torch, seed=2).cuda()
+    nb = 24
+    with pytest.raises(czw.CZWError):  # records on the host
+        czw.compress_field(f, 1e-3, 3, records=torch.empty(czw.records_bound(nb, 4), dtype=torch.uint8))
+    with pytest.raises(czw.CZWError):  # offsets one short
+        czw.compress_field(f, 1e-3, 3, offsets=torch.empty(nb, dtype=torch.int64, device="cuda"))
+    with pytest.raises(czw.CZWError):  # workspace too small
+        czw.compress_field(f, 1e-3, 3, workspace=torch.empty(16, dtype=torch.uint8, device="cuda"))
+    rec, off = czw.compress_field(f, 1e-3, 3)
+    with pytest.raises(czw.CZWError):  # output of another shape
+        czw.decompress_field(rec, off, SHAPE, torch.float32, 3, out=torch.empty((64, 96, 96), device="cuda"))
+    with pytest.raises(czw.CZWError):  # output of another dtype
+        czw.decompress_field(rec, off, SHAPE, torch.float32, 3, out=torch.empty(SHAPE, dtype=torch.float64,
+                                                                                  device="cuda"))
+    with pytest.raises(czw.CZWError):  # offsets too short for the field
+        czw.decompress_field(rec, off[:10], SHAPE, torch.float32, 3)
+    with pytest.raises(czw.CZWError):  # size array too short for the slots stage
+        czw.compress_field_slots(f, 1e-3, 3, 0, torch.empty(nb * czw.slot_bytes(4), dtype=torch.uint8, device="cuda"),
+                                 torch.empty(nb, dtype=torch.int32, device="cuda"))
+    out = czw.decompress_field(rec, off, SHAPE, torch.float32, 3)
+    assert out.shape == SHAPE
+
+
+@pytest.mark.parametrize("prec", [4, 8])
+def test_two_stage_equals_one_call(torch_gpu, prec):
+    """compress_field_slots + pack_records (the two stages timed apart) give
+    the records of compress_field."""
+    torch = torch_gpu
+    print(f"[case] two-stage prec={prec}", flush=True)
+    f = _field(torch, seed=8, dtype=torch.float32 if prec == 4 else torch.float64).cuda()
+    nb = 24
+    rec, off = czw.compress_field(f, 1e-4, 1, 0, True)
+    slots = torch.empty(nb * czw.slot_bytes(prec), dtype=torch.uint8, device="cuda")
+    nbytes = torch.empty(czw.size_words(prec) * nb, dtype=torch.int32, device="cuda")
+    rec2 = torch.empty(czw.records_bound(nb, prec), dtype=torch.uint8, device="cuda")
+    off2 = torch.empty(nb + 1, dtype=torch.int64, device="cuda")
+    czw.compress_field_slots(f, 1e-4, 1, 0, slots, nbytes)
+    czw.pack_records(slots, czw.slot_bytes(prec), nbytes, nb, True, rec2, off2)
+    assert torch.equal(off, off2) and torch.equal(rec[: int(off[-1])], rec2[: int(off2[-1])])
