@@ -69,3 +69,19 @@ def test_device_writer_other_encoders(torch_gpu, tmp_path, dt, encoder):
         assert cf.encoder == encoder
         r2, o2 = cf.records()
     assert np.array_equal(r2, r) and np.array_equal(o2, o)
+
+
+def test_device_writer_size_only_and_bad_path(torch_gpu, tmp_path):
+    """path None reports the size the file would have; a path that cannot be
+    opened is an error before any device work, and leaves no file."""
+    torch = torch_gpu
+    print("[case] device writer size-only / bad path", flush=True)
+    f = fields.bubble_field(SHAPE, seed=10, nbubbles=5, dtype=torch.float32, device="cuda")
+    got = tmp_path / "got.cz"
+    nbytes, _ = czfile.compress_device_field(str(got), f, 1e-4, 1)
+    size_only, _ = czfile.compress_device_field(None, f, 1e-4, 1)
+    assert size_only == nbytes == got.stat().st_size
+    bad = tmp_path / "missing_dir" / "x.cz"
+    with pytest.raises(czfile.CzFileError):
+        czfile.compress_device_field(str(bad), f, 1e-4, 1)
+    assert not bad.exists()
