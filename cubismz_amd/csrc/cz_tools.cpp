@@ -167,8 +167,10 @@ bool load_field(const Args &a, Field &f)
 }
 
 // Decode a whole .cz into a field: every chunk inflated once (czh gather),
-// all wavelet blocks decoded by one device launch.
-bool decode_cz(const std::string &path, int expect_wtype, Field &f, bool verbose)
+// all wavelet blocks decoded by one device launch.  The file does not record
+// the byte shuffle (the reference selects it at compile time, _USE_SHUFFLE3_,
+// for its writer and reader alike), so the caller says whether it was used.
+bool decode_cz(const std::string &path, int expect_wtype, Field &f, bool verbose, bool shuffle)
 {
     std::vector<uint8_t> bytes;
     cz::FileInfo info;
@@ -231,8 +233,8 @@ bool decode_cz(const std::string &path, int expect_wtype, Field &f, bool verbose
     HIPCHECK(hipMemcpy(drec, rec.data(), rec.size(), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(doff, off.data(), off.size() * 8, hipMemcpyHostToDevice));
     HIPCHECK(hipMemset(derr, 0, 4));
-    const int rc = czw_decompress_field(drec, rec.size(), doff, f.n[0], f.n[1], f.n[2], f.prec, p.wtype, 0, dfield,
-                                        derr, 0);
+    const int rc = czw_decompress_field(drec, rec.size(), doff, f.n[0], f.n[1], f.n[2], f.prec, p.wtype, shuffle ? 1 : 0,
+                                        dfield, derr, 0);
     if (rc != CZW_OK) {
         std::fprintf(stderr, "czw_decompress_field failed (%d)\n", rc);
         return false;
@@ -391,13 +393,15 @@ int tool_main(int argc, char **argv)
     Args a(argc, argv);
     const std::string in = a.str("-czfile", "none");
     if (a.has("-help") || in == "none" || (!a.has("-h5file") && !a.has("-rawfile"))) {
-        std::printf("Usage: %s -czfile <cz file> -h5file <h5 basefilename> | -rawfile <raw file> [-wtype <wt>]\n",
+        std::printf("Usage: %s -czfile <cz file> -h5file <h5 basefilename> | -rawfile <raw file> [-wtype <wt>] "
+                    "[-shuffle]\n",
                     argv[0]);
         return 1;
     }
     const double t0 = now();
     Field f;
-    if (!decode_cz(in, a.i("-wtype", -1), f, true)) return 1;
+    const bool shuffle = a.has("-shuffle") && a.str("-shuffle", "1") != "0";
+    if (!decode_cz(in, a.i("-wtype", -1), f, true, shuffle)) return 1;
     std::printf("I found in total %dx%dx%d blocks.\n", f.n[0] / 32, f.n[1] / 32, f.n[2] / 32);
     if (a.has("-rawfile")) {
         if (!cz::write_file(a.str("-rawfile", ""), f.data)) return 1;
@@ -446,17 +450,19 @@ int tool_main(int argc, char **argv)
     Args a(argc, argv);
     const std::string f1n = a.str("-czfile1", "none"), f2n = a.str("-czfile2", "none");
     if (a.has("-help") || f1n == "none" || (f2n == "none" && !a.has("-rawfile2"))) {
-        std::printf("Usage: %s -czfile1 <cz file1> [-wtype <wt>] -czfile2 <cz reference file2>\n", argv[0]);
+        std::printf("Usage: %s -czfile1 <cz file1> [-wtype <wt>] -czfile2 <cz reference file2> [-shuffle]\n",
+                    argv[0]);
         return 1;
     }
     const double init_t0 = now();
     Field A, B;
-    if (!decode_cz(f1n, a.i("-wtype", 3), A, false)) return 1;
+    const bool shuffle = a.has("-shuffle") && a.str("-shuffle", "1") != "0";  // both files, as in one build
+    if (!decode_cz(f1n, a.i("-wtype", 3), A, false, shuffle)) return 1;
     if (a.has("-rawfile2")) {
         B.prec = A.prec;
         std::memcpy(B.n, A.n, sizeof(B.n));
         if (!read_raw(a.str("-rawfile2", ""), B)) return 1;
-    } else if (!decode_cz(f2n, -1, B, false)) {
+    } else if (!decode_cz(f2n, -1, B, false, shuffle)) {
         return 1;
     }
     const double init_t1 = now();

@@ -146,3 +146,25 @@ def test_hdf5_write_and_read(demo_raw, tmp_path):
         blk = rec[o + 4:o + 4 + 32768 * 4].view(np.float32).reshape(32, 32, 32)  # [z][y][x]
         got[32 * ix:32 * ix + 32, 32 * iy:32 * iy + 32, 32 * iz:32 * iz + 32] = blk.transpose(2, 1, 0)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_shuffled_file_needs_and_takes_the_shuffle_flag(demo_raw, tmp_path):
+    """The .cz header does not record the byte shuffle (the reference builds
+    writer and reader with the same _USE_SHUFFLE3_); cz2hdf/cz2diff -shuffle
+    decode a shuffled file to exactly the values of the unshuffled one."""
+    common = ["-rawfile", str(demo_raw), "-bpdx", "4", "-bpdy", "4", "-bpdz", "4", "-fpath", str(tmp_path),
+              "-threshold", "0.00005", "-wtype", "3", "-zerobits", "4"]
+    _tool("hdf2cz", *common, "-czfile", "sh.cz", "-shuffle")
+    _tool("hdf2cz", *common, "-czfile", "ns.cz")
+    _tool("cz2hdf", "-czfile", str(tmp_path / "sh.cz"), "-rawfile", str(tmp_path / "sh.raw"), "-shuffle")
+    _tool("cz2hdf", "-czfile", str(tmp_path / "ns.cz"), "-rawfile", str(tmp_path / "ns.raw"))
+    assert (tmp_path / "sh.raw").read_bytes() == (tmp_path / "ns.raw").read_bytes()
+    _tool("cz2hdf", "-czfile", str(tmp_path / "sh.cz"), "-rawfile", str(tmp_path / "wrong.raw"))
+    assert (tmp_path / "wrong.raw").read_bytes() != (tmp_path / "ns.raw").read_bytes()
+    res = []
+    for name, flag in (("sh.cz", ["-shuffle"]), ("ns.cz", [])):
+        out = _tool("cz2diff", "-czfile1", str(tmp_path / name), "-wtype", "3", "-rawfile2", str(demo_raw), *flag)
+        res.append([l for l in out.splitlines() if l.startswith("RES:")][-1].split())
+    # RES: ratio, five error norms, bits per sample, PSNR; the shuffle changes
+    # only the file size (ratio, bits per sample)
+    assert res[0][2:7] == res[1][2:7] and res[0][8] == res[1][8], (res[0], res[1])
