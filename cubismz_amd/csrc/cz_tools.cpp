@@ -140,11 +140,12 @@ bool read_h5(const std::string &path, const std::string &dset, int channel, Fiel
 }
 
 // Write dataset "data" with dims (NX,NY,NZ,1) (Tools/cz2hdf.cpp:158-199).
-bool write_h5(const std::string &path, const Field &f)
+// /data as [NX][NY][NZ][nch] (the field's data holds nch values per point).
+bool write_h5(const std::string &path, const Field &f, int nch = 1)
 {
     hid_t file = H5Fcreate(path.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
     if (file < 0) return false;
-    hsize_t dims[4] = {(hsize_t)f.n[0], (hsize_t)f.n[1], (hsize_t)f.n[2], 1};
+    hsize_t dims[4] = {(hsize_t)f.n[0], (hsize_t)f.n[1], (hsize_t)f.n[2], (hsize_t)nch};
     hid_t sp = H5Screate_simple(4, dims, NULL);
     const hid_t mt = f.prec == 4 ? H5T_NATIVE_FLOAT : H5T_NATIVE_DOUBLE;
     hid_t ds = H5Dcreate2(file, "data", mt, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
@@ -156,7 +157,7 @@ bool write_h5(const std::string &path, const Field &f)
 }
 #endif
 
-bool load_field(const Args &a, Field &f)
+[[maybe_unused]] bool load_field(const Args &a, Field &f)
 {
     if (a.has("-rawfile")) return read_raw(a.str("-rawfile", ""), f);
 #ifdef CZ_HAVE_HDF5
@@ -386,50 +387,96 @@ int tool_main(int argc, char **argv)
     return 0;
 }
 #elif CZ_TOOL == 2
-// cz2hdf (Tools/cz2hdf.cpp): decode a .cz to <name>.h5 + <name>.h5.xmf, or to
-// a raw file with -rawfile.
+// cz2hdf (Tools/cz2hdf.cpp): decode up to three .cz files (-czfile,
+// -czfile2, -czfile3: the channels of /data, cz2hdf.cpp:51-70) over a region
+// of interest of whole blocks (-xs/-xe/-ys/-ye/-zs/-ze, block indices, -1 =
+// the whole extent, :72-153) to <name>.h5 + <name>.xmf (/data is
+// [NX][NY][NZ][channels], the reference's _TRANSPOSE_DATA_ layout), or to a
+// raw file of the same layout with -rawfile.
 int tool_main(int argc, char **argv)
 {
     Args a(argc, argv);
-    const std::string in = a.str("-czfile", "none");
-    if (a.has("-help") || in == "none" || (!a.has("-h5file") && !a.has("-rawfile"))) {
+    const std::string in[3] = {a.str("-czfile", "none"), a.str("-czfile2", "none"), a.str("-czfile3", "none")};
+    if (a.has("-help") || in[0] == "none" || (!a.has("-h5file") && !a.has("-rawfile"))) {
         std::printf("Usage: %s -czfile <cz file> -h5file <h5 basefilename> | -rawfile <raw file> [-wtype <wt>] "
-                    "[-shuffle]\n",
+                    "[-czfile2 <cz file>] [-czfile3 <cz file>] [-xs/-xe/-ys/-ye/-zs/-ze <block>] [-shuffle]\n",
                     argv[0]);
         return 1;
     }
+    if (a.has("-swap")) {
+        std::fprintf(stderr, "cz2hdf: -swap (big-endian files) is not supported\n");
+        return 1;
+    }
     const double t0 = now();
-    Field f;
     const bool shuffle = a.has("-shuffle") && a.str("-shuffle", "1") != "0";
-    if (!decode_cz(in, a.i("-wtype", -1), f, true, shuffle)) return 1;
-    std::printf("I found in total %dx%dx%d blocks.\n", f.n[0] / 32, f.n[1] / 32, f.n[2] / 32);
+    Field ch[3];
+    int nch = 0;
+    for (int c = 0; c < 3; ++c) {
+        if (in[c] == "none") continue;
+        if (!decode_cz(in[c], a.i("-wtype", -1), ch[nch], true, shuffle)) return 1;
+        if (nch && (std::memcmp(ch[nch].n, ch[0].n, sizeof(ch[0].n)) || ch[nch].prec != ch[0].prec)) {
+            std::fprintf(stderr, "cz2hdf: %s differs in extent or precision from %s\n", in[c].c_str(), in[0].c_str());
+            return 1;
+        }
+        ++nch;
+    }
+    const Field &f0 = ch[0];
+    const int nbk[3] = {f0.n[0] / 32, f0.n[1] / 32, f0.n[2] / 32};
+    std::printf("I found in total %dx%dx%d blocks.\n", nbk[0], nbk[1], nbk[2]);
+    const char *ks[3] = {"-xs", "-ys", "-zs"}, *ke[3] = {"-xe", "-ye", "-ze"};
+    int st[3], en[3];
+    for (int d = 0; d < 3; ++d) {
+        const int s0 = a.i(ks[d], -1), e0 = a.i(ke[d], -1);
+        st[d] = s0 == -1 ? 0 : s0;
+        en[d] = e0 == -1 ? nbk[d] - 1 : e0;
+        if (st[d] < 0 || en[d] >= nbk[d] || st[d] > en[d]) {
+            std::fprintf(stderr, "cz2hdf: empty or out-of-range region in %c\n", "xyz"[d]);
+            return 1;
+        }
+    }
+    std::printf("ROI = [%d,%d]x[%d,%d]x[%d,%d]\n", st[0], en[0], st[1], en[1], st[2], en[2]);
+    Field out;
+    out.prec = f0.prec;
+    for (int d = 0; d < 3; ++d) out.n[d] = (en[d] - st[d] + 1) * 32;
+    out.data.resize(out.count() * nch * out.prec);
+    const size_t p = out.prec;
+    for (int x = 0; x < out.n[0]; ++x)
+        for (int y = 0; y < out.n[1]; ++y)
+            for (int c = 0; c < nch; ++c) {
+                const size_t src = ((size_t)(x + 32 * st[0]) * f0.n[1] + (y + 32 * st[1])) * f0.n[2] + 32 * st[2];
+                uint8_t *dst = out.data.data() + (((size_t)x * out.n[1] + y) * out.n[2] * nch + c) * p;
+                const uint8_t *s = ch[c].data.data() + src * p;
+                for (int z = 0; z < out.n[2]; ++z) std::memcpy(dst + (size_t)z * nch * p, s + (size_t)z * p, p);
+            }
     if (a.has("-rawfile")) {
-        if (!cz::write_file(a.str("-rawfile", ""), f.data)) return 1;
+        if (!cz::write_file(a.str("-rawfile", ""), out.data)) return 1;
     } else {
 #ifdef CZ_HAVE_HDF5
         const std::string base = a.str("-h5file", "out");
         const std::string h5 = base + ".h5";
-        if (!write_h5(h5, f)) {
+        if (!write_h5(h5, out, nch)) {
             std::fprintf(stderr, "cz2hdf: writing %s failed\n", h5.c_str());
             return 1;
         }
-        FILE *xmf = std::fopen((h5 + ".xmf").c_str(), "w");  // cz2hdf.cpp:318-374
+        FILE *xmf = std::fopen((base + ".xmf").c_str(), "w");  // cz2hdf.cpp:318-374
         if (xmf) {
-            const int mx = std::max(f.n[0], std::max(f.n[1], f.n[2]));
-            const char *nt = f.prec == 4 ? "Float\" Precision=\"4" : "Double\" Precision=\"8";
+            const int mx = std::max(out.n[0], std::max(out.n[1], out.n[2]));
+            const char *nt = out.prec == 4 ? "Float\" Precision=\"4" : "Double\" Precision=\"8";
             std::string name = h5.substr(h5.find_last_of('/') == std::string::npos ? 0 : h5.find_last_of('/') + 1);
             std::fprintf(xmf, "<?xml version=\"1.0\" ?>\n<!DOCTYPE Xdmf SYSTEM \"Xdmf.dtd\" []>\n<Xdmf Version=\"2.0\">\n");
             std::fprintf(xmf, " <Domain>\n   <Grid GridType=\"Uniform\">\n     <Time Value=\"%05d\"/>\n", 0);
-            std::fprintf(xmf, "     <Topology TopologyType=\"3DCORECTMesh\" Dimensions=\"%d %d %d\"/>\n", f.n[0], f.n[1], f.n[2]);
+            std::fprintf(xmf, "     <Topology TopologyType=\"3DCORECTMesh\" Dimensions=\"%d %d %d\"/>\n", out.n[0], out.n[1],
+                         out.n[2]);
             std::fprintf(xmf, "     <Geometry GeometryType=\"ORIGIN_DXDYDZ\">\n");
             std::fprintf(xmf, "       <DataItem Name=\"Origin\" Dimensions=\"3\" NumberType=\"%s\" Format=\"XML\">\n", nt);
             std::fprintf(xmf, "        %e %e %e\n       </DataItem>\n", 0., 0., 0.);
             std::fprintf(xmf, "       <DataItem Name=\"Spacing\" Dimensions=\"3\" NumberType=\"%s\" Format=\"XML\">\n", nt);
-            const double h = f.prec == 4 ? (double)(1.f / (float)mx) : 1.0 / (double)mx;
+            const double h = out.prec == 4 ? (double)(1.f / (float)mx) : 1.0 / (double)mx;
             std::fprintf(xmf, "        %e %e %e\n       </DataItem>\n     </Geometry>\n", h, h, h);
-            std::fprintf(xmf, "     <Attribute Name=\"data\" AttributeType=\"%s\" Center=\"Node\">\n", "Scalar");
+            std::fprintf(xmf, "     <Attribute Name=\"data\" AttributeType=\"%s\" Center=\"Node\">\n",
+                         nch == 1 ? "Scalar" : "Vector");
             std::fprintf(xmf, "       <DataItem Dimensions=\"%d %d %d %d\" NumberType=\"Float\" Precision=\"%d\" Format=\"HDF\">\n",
-                         f.n[0], f.n[1], f.n[2], 1, f.prec);
+                         out.n[0], out.n[1], out.n[2], nch, out.prec);
             std::fprintf(xmf, "        %s:/data\n       </DataItem>\n     </Attribute>\n   </Grid>\n </Domain>\n</Xdmf>\n",
                          name.c_str());
             std::fclose(xmf);

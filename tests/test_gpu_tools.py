@@ -132,7 +132,7 @@ def test_hdf5_write_and_read(demo_raw, tmp_path):
           "-fpath", str(tmp_path), "-threshold", "0.00005", "-wtype", "3")
     _tool("cz2hdf", "-czfile", str(tmp_path / "tmp.cz"), "-h5file", str(tmp_path / "out"))
     _tool("cz2hdf", "-czfile", str(tmp_path / "tmp.cz"), "-rawfile", str(tmp_path / "out.raw"))
-    assert (tmp_path / "out.h5").exists() and (tmp_path / "out.h5.xmf").exists()
+    assert (tmp_path / "out.h5").exists() and (tmp_path / "out.xmf").exists()
     _tool("hdf2cz", "-h5file", str(tmp_path / "out.h5"), "-bpdx", "4", "-bpdy", "4", "-bpdz", "4", "-czfile", "r.cz",
           "-fpath", str(tmp_path), "-nowavelets", "-encoder", "none")
     want = np.fromfile(tmp_path / "out.raw", np.float32).reshape(128, 128, 128)
@@ -168,3 +168,22 @@ def test_shuffled_file_needs_and_takes_the_shuffle_flag(demo_raw, tmp_path):
     # RES: ratio, five error norms, bits per sample, PSNR; the shuffle changes
     # only the file size (ratio, bits per sample)
     assert res[0][2:7] == res[1][2:7] and res[0][8] == res[1][8], (res[0], res[1])
+
+
+def test_cz2hdf_channels_and_region(demo_raw, tmp_path):
+    """cz2hdf -czfile2 stacks a second file as channel 1 of /data, and
+    -xs/-xe/-ys/-ye/-zs/-ze crop to whole blocks (cz2hdf.cpp:51-153)."""
+    common = ["-rawfile", str(demo_raw), "-bpdx", "4", "-bpdy", "4", "-bpdz", "4", "-fpath", str(tmp_path),
+              "-threshold", "0.00005"]
+    _tool("hdf2cz", *common, "-czfile", "a.cz", "-wtype", "3")
+    _tool("hdf2cz", *common, "-czfile", "b.cz", "-wtype", "1")
+    full = {}
+    for n in ("a", "b"):
+        _tool("cz2hdf", "-czfile", str(tmp_path / f"{n}.cz"), "-rawfile", str(tmp_path / f"{n}.raw"))
+        full[n] = np.fromfile(tmp_path / f"{n}.raw", np.float32).reshape(128, 128, 128)
+    out = _tool("cz2hdf", "-czfile", str(tmp_path / "a.cz"), "-czfile2", str(tmp_path / "b.cz"), "-xs", "1", "-xe", "2",
+                "-ys", "0", "-ye", "1", "-zs", "3", "-ze", "3", "-rawfile", str(tmp_path / "roi.raw"))
+    assert "ROI = [1,2]x[0,1]x[3,3]" in out
+    roi = np.fromfile(tmp_path / "roi.raw", np.float32).reshape(64, 64, 32, 2)
+    assert np.array_equal(roi[..., 0], full["a"][32:96, 0:64, 96:128])
+    assert np.array_equal(roi[..., 1], full["b"][32:96, 0:64, 96:128])
