@@ -89,6 +89,27 @@ def profiled_traffic(kernel):
     return None
 
 
+def fp64_comparison(a, torch, czw, field, out32, rec32, off32, shape, nblocks):
+    """The same input through the binary64 path: how far the float
+    reconstruction lies from the double one, and how many keep decisions
+    (mask bits) the two precisions take differently."""
+    fd = field.double()
+    rec64, off64 = czw.compress_field(fd, a.eps, a.wtype, a.zerobits, bool(a.shuffle))
+    out64 = czw.decompress_field(rec64, off64, shape, torch.float64, a.wtype, bool(a.shuffle))
+    d = out32.double() - out64
+    res = {"max_abs_diff": float(d.abs().max().item()),
+           "rel_l2_diff": float((d.norm() / out64.norm()).item()),
+           "linf_fp64_vs_input": float((out64 - fd).abs().max().item())}
+    del d, fd, out64
+    idx = torch.arange(4096, device=rec32.device, dtype=torch.int64)
+    m32 = rec32[(off32[:-1, None] + 4 + idx).reshape(-1)]
+    m64 = rec64[(off64[:-1, None] + 4 + idx).reshape(-1)]
+    lut = torch.tensor([bin(i).count("1") for i in range(256)], dtype=torch.int64, device=rec32.device)
+    res["mask_bits_differing"] = int(lut[torch.bitwise_xor(m32, m64).long()].sum().item())
+    res["mask_bits"] = nblocks * 32768
+    return res
+
+
 def _events(torch, n):
     return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
 
@@ -502,6 +523,7 @@ def main():
     err_inf = float(np.max(np.abs(outh.astype(np.float64) - fh)))
     mse = float(np.mean((outh.astype(np.float64) - fh) ** 2))
     psnr = 20 * np.log10((fh.max() - fh.min()) / (2 * np.sqrt(mse))) if mse > 0 else float("inf")
+    fp64 = fp64_comparison(a, torch, czw, field, out, records, offsets, shape, nblocks)
 
     cpu = None
     parity = None
@@ -546,13 +568,15 @@ def main():
             cpu["one_thread"] = {"value": round(field_bytes / (tr2 - tr0) / 1e9, 4), "unit": "GB/s",
                                  "sample": f"whole {n}^3 field once, compress {tr1 - tr0:.3f} s + "
                                            f"decompress {tr2 - tr1:.3f} s"}
-        mism = 0
+        mism = flips = 0
         for b in range(nblocks):
             o = int(off[b])
             if int(recs[o:o + 4].view(np.int32)[0]) != int(onb[b]) or \
                     not np.array_equal(recs[o + 4:o + 4 + int(onb[b])], oslots[b, :onb[b]]):
                 mism += 1
+            flips += int(np.unpackbits(np.bitwise_xor(recs[o + 4:o + 4 + 4096], oslots[b, :4096])).sum())
         parity = {"blocks": nblocks, "records_bitexact": mism == 0, "blocks_differing": mism,
+                  "mask_bits_flipped": flips,
                   "decoded_bitexact": bool(np.array_equal(outh.view(np.uint32), ofield.view(np.uint32)))}
 
     res = {
@@ -586,7 +610,7 @@ def main():
         "f64_scaleup": f64,
         "compression": {"payload_bytes": payload, "survivor_fraction": round(rho, 5),
                         "ratio_vs_records": round(field_bytes / float(off[-1]), 3),
-                        "linf": err_inf, "psnr_db": round(float(psnr), 4)},
+                        "linf": err_inf, "psnr_db": round(float(psnr), 4), "vs_fp64_path": fp64},
         "cpu_baseline": cpu,
         "parity": parity,
     }
