@@ -28,6 +28,7 @@ Also reported:
                 oracle for every block.
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -57,6 +58,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the full parity check")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-mode leg (64 uneven fields)")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the eps x zerobits sweep of the 512^3 field and the 128^3 config-0 case")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg (8 x 128^3)")
     ap.add_argument("--no-f64", action="store_true", help="skip the binary64 scale-up leg (1024^3 f64)")
     ap.add_argument("--f64-size", type=int, default=1024)
@@ -87,6 +90,35 @@ def profiled_traffic(kernel):
                 if key in name and (("<float" in name) == ("<float" in kernel)):
                     return int(v)
     return None
+
+
+def config0_leg(a, torch, czw, fields):
+    """The reference's CPU-sized case: a 128^3 float32 field (1.0 + 16
+    Gaussian bumps + a tanh interface + noise, seed 0), the fourth-order
+    interpolating wavelet, eps 1e-3, byte shuffle, zlib: the .cz file's
+    ratio and PSNR, and the decode."""
+    import tempfile
+
+    import numpy as np
+
+    from cubismz_amd import czfile
+
+    f = fields.bumps_field((128, 128, 128), seed=0, dtype=torch.float32).cuda()
+    rec, off = czw.compress_field(f, 1e-3, 1, 0, True)
+    dec = czw.decompress_field(rec, off, f.shape, torch.float32, 1, True)
+    o = off.cpu().numpy()
+    r = rec[: int(o[-1])].cpu().numpy()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "c0.cz")
+        czfile.write(path, r, o, (4, 4, 4), 4, 1, 1e-3, encoder="zlib")
+        file_bytes = os.path.getsize(path)
+    fh, dh = f.cpu().numpy().astype(np.float64), dec.cpu().numpy()
+    e = dh.astype(np.float64) - fh
+    mse = float(np.mean(e ** 2))
+    return {"workload": "128^3 f32 bumps field (seed 0), wavelet type 1, eps 1e-3, shuffle, zlib",
+            "file_bytes": file_bytes, "ratio": round(fh.size * 4 / file_bytes, 3),
+            "psnr_db": round(20 * float(np.log10((fh.max() - fh.min()) / (2 * np.sqrt(mse)))), 4),
+            "linf": float(np.abs(e).max()), "_host": (fh.astype(np.float32), r, o, dh)}
 
 
 def fp64_comparison(a, torch, czw, field, out32, rec32, off32, shape, nblocks):
@@ -405,6 +437,11 @@ def main():
         if ev:
             ev[3].record()
 
+    # The launches come from Python, so the collector stays off from here to
+    # the end of the timed steps (as timeit does): a collection pause leaves
+    # the GPU idle, and after idle the clocks need milliseconds to ramp up.
+    gc.collect()
+    gc.disable()
     settle = 0
     t_settle = time.perf_counter()
     while time.perf_counter() - t_settle < a.settle_ms * 1e-3 and settle < 20000:
@@ -456,6 +493,7 @@ def main():
     g1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    gc.enable()
     t_step = g0.elapsed_time(g1) / a.steps * 1e-3
     ms_per_step = wall / a.steps * 1e3
     graph_ok = None
@@ -496,6 +534,53 @@ def main():
         torch.cuda.synchronize()
         tw = e0.elapsed_time(e1) / a.steps * 1e-3
         other[f"wtype{w}"] = {"GBps": round(field_bytes / tw / 1e9, 3), "ms_per_step": round(tw * 1e3, 4)}
+    # the 512^3 configuration's other settings: eps in {1e-2, 1e-3, 1e-4} x
+    # zerobits {0, 4} (records kept for the parity check below)
+    sweep = {}
+    sweep_recs = {}
+    if not a.no_sweep:
+        settings = [(eps, zb) for eps in (1e-2, 1e-3, 1e-4) for zb in (0, 4)]
+
+        def sw(eps, zb):
+            czw.compress_field_slots(field, eps, a.wtype, zb, slots, nbytes)
+            czw.pack_records(slots, slot, nbytes, nblocks, a.shuffle, records, offsets)
+            czw.decompress_field(records, offsets, shape, torch.float32, a.wtype, a.shuffle, out=out, err=err,
+                                 check=False)
+
+        # timed back to back, so that the GPU never idles between settings
+        # (clocks drop within milliseconds of idle); the best of three windows,
+        # so that a host-side stall in one (the launches are issued from
+        # Python) does not count
+        reps = max(10, a.steps // 4)
+        gc.collect()
+        gc.disable()
+        for eps, zb in settings:
+            for _ in range(10):
+                sw(eps, zb)
+            tws = []
+            for _ in range(3):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    sw(eps, zb)
+                e1.record()
+                e1.synchronize()
+                tws.append(e0.elapsed_time(e1) / reps * 1e-3)
+            tw = min(tws)
+            sweep[f"eps{eps:g}_zb{zb}"] = {"GBps": round(field_bytes / tw / 1e9, 3), "ms_per_step": round(tw * 1e3, 4),
+                                           "ms_per_step_windows": [round(x * 1e3, 4) for x in tws],
+                                           "survivor_fraction": round(float(offsets[-1].item() - 4100 * nblocks) /
+                                                                      (4 * nblocks * 32768), 5)}
+        gc.enable()
+        if int(err.item()):
+            raise SystemExit("decoder reported corrupt records in the sweep")
+        if not a.no_cpu:  # each setting's records and decode, for the parity check below
+            for eps, zb in settings:
+                sw(eps, zb)
+                torch.cuda.synchronize()
+                sweep_recs[f"eps{eps:g}_zb{zb}"] = (eps, zb, records[: int(offsets[-1].item())].cpu().numpy(),
+                                                    offsets.cpu().numpy(), out.cpu().numpy())
+    config0 = None if a.no_sweep else config0_leg(a, torch, czw, fields)
     batched = None if a.no_batch else batch_leg(a, torch, czw, fields)
     adjoint = None if a.no_adjoint else adjoint_leg(a, torch, fields)
     f64 = None if a.no_f64 else f64_leg(a, torch, czw, fields)
@@ -578,6 +663,19 @@ def main():
         parity = {"blocks": nblocks, "records_bitexact": mism == 0, "blocks_differing": mism,
                   "mask_bits_flipped": flips,
                   "decoded_bitexact": bool(np.array_equal(outh.view(np.uint32), ofield.view(np.uint32)))}
+        for key, (eps, zb, rr, oo, dd) in sweep_recs.items():
+            sl, nbs = O.compress_field(fh, eps, a.wtype, zb, bool(a.shuffle))
+            same = all(int(rr[oo[b]:oo[b] + 4].view(np.int32)[0]) == int(nbs[b]) and
+                       np.array_equal(rr[oo[b] + 4:oo[b] + 4 + int(nbs[b])], sl[b, :nbs[b]]) for b in range(nblocks))
+            od = O.decompress_field(sl, nbs, shape, a.wtype, "f32", bool(a.shuffle))
+            sweep[key]["bitexact"] = bool(same and np.array_equal(dd.view(np.uint32), od.view(np.uint32)))
+            del sl, od
+        if config0 is not None:
+            c0f, c0r, c0o, c0d = config0["_host"]
+            sl, nbs = O.compress_field(c0f, 1e-3, 1, 0, True)
+            same = all(np.array_equal(c0r[c0o[b] + 4:c0o[b + 1]], sl[b, :nbs[b]]) for b in range(64))
+            od = O.decompress_field(sl, nbs, c0f.shape, 1, "f32", True)
+            config0["bitexact"] = bool(same and np.array_equal(c0d.view(np.uint32), od.view(np.uint32)))
 
     res = {
         "metric": "GB/s compress+decompress 512^3 fp32 eps=1e-3 (bit-exact); batched & adjoint GB/s",
@@ -605,6 +703,8 @@ def main():
                      "alg_bytes_per_launch": kern[dom][0], "avg_launch_ms": round(kern[dom][1] * 1e3, 4)},
         "step_roofline_frac": round((alg_c + alg_d) / t_step / HBM_PEAK, 4),
         "other_wtypes": other,
+        "eps_zerobits_sweep": sweep or None,
+        "config0": {k: v for k, v in config0.items() if k != "_host"} if config0 else None,
         "batched": batched,
         "adjoint": adjoint,
         "f64_scaleup": f64,
