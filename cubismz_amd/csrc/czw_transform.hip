@@ -1,6 +1,8 @@
 // czw_transform.hip -- whole-block multilevel transforms (forward, inverse and
 // the adjoint of the inverse) on blocks in the kernels' orientation
-// O[z][y][x], in binary32 or binary64, one launch per 1D pass.
+// O[z][y][x]: binary32 blocks in one launch, each block held in LDS for all
+// nine passes; binary64 (a 256 KB block does not fit in LDS) one launch per
+// 1D pass.
 //
 // Used by the differentiable inverse (cubismz_amd.autograd): the inverse
 // transform with a frozen survivor mask is linear in the coefficients, and
@@ -46,6 +48,86 @@ __global__ __launch_bounds__(256) void k_rowpass(R *__restrict__ blocks, int nbl
     for (int k = 0; k < N; ++k) p[k * stride] = v[k];
 }
 
+// Binary32: one workgroup per block.  The block is read once into LDS
+// (O[z][y][x] at z*TSP + y*TRP + x, the padding that makes rows along x, y
+// and z conflict-free), all nine passes of the schedule run there with the
+// same row functions and order as k_rowpass, and the block is written back
+// once: 256 KB of HBM traffic per block instead of nine times that.
+constexpr int TRP = 33, TSP = 32 * 33 + 1, TT = 512;
+
+template <int WT, int OP, int N>
+CZW_DI void lds_pass(float *A, int t, int ax)
+{
+    const int stride = ax == 0 ? TSP : (ax == 1 ? TRP : 1);
+    const int si = ax == 0 ? TRP : TSP, sj = ax == 2 ? TRP : 1;  // the other two coordinates, slower first
+    constexpr int RPT = (N * N + TT - 1) / TT;                   // rows per thread
+    float v[RPT][N];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {  // every row loaded before the first is written back
+        const int r = t + TT * q;
+        if (r < N * N) {
+            const float *p = A + (r / N) * si + (r % N) * sj;
+#pragma unroll
+            for (int k = 0; k < N; ++k) v[q][k] = p[k * stride];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = t + TT * q;
+        if (r < N * N) {
+            if constexpr (OP == OP_FWD) fwd_row<N, WT, float>(v[q]);
+            else if constexpr (OP == OP_INV) inv_row<N, WT, float>(v[q]);
+            else adj_inv_row<N, WT, float>(v[q]);
+            float *p = A + (r / N) * si + (r % N) * sj;
+#pragma unroll
+            for (int k = 0; k < N; ++k) p[k * stride] = v[q][k];
+        }
+    }
+    __syncthreads();
+}
+
+template <int WT, int OP>
+__global__ __launch_bounds__(TT) void k_transform_lds_f32(float *__restrict__ blocks)
+{
+    __shared__ float A[32 * TSP];
+    const int t = threadIdx.x;
+    float *g = blocks + (size_t)blockIdx.x * 32768;
+#pragma unroll 8
+    for (int k = 0; k < 32768 / TT; ++k) {
+        const int e = t + TT * k;
+        A[(e >> 10) * TSP + ((e >> 5) & 31) * TRP + (e & 31)] = g[e];
+    }
+    __syncthreads();
+    enum { Z = 0, Y = 1, X = 2 };
+    if constexpr (OP == OP_FWD) {  // level 32: x, y, z; 16: z, x, y; 8: y, z, x
+        lds_pass<WT, OP, 32>(A, t, X); lds_pass<WT, OP, 32>(A, t, Y); lds_pass<WT, OP, 32>(A, t, Z);
+        lds_pass<WT, OP, 16>(A, t, Z); lds_pass<WT, OP, 16>(A, t, X); lds_pass<WT, OP, 16>(A, t, Y);
+        lds_pass<WT, OP, 8>(A, t, Y); lds_pass<WT, OP, 8>(A, t, Z); lds_pass<WT, OP, 8>(A, t, X);
+    } else if constexpr (OP == OP_INV) {  // level 8: x, z, y; 16: y, x, z; 32: z, y, x
+        lds_pass<WT, OP, 8>(A, t, X); lds_pass<WT, OP, 8>(A, t, Z); lds_pass<WT, OP, 8>(A, t, Y);
+        lds_pass<WT, OP, 16>(A, t, Y); lds_pass<WT, OP, 16>(A, t, X); lds_pass<WT, OP, 16>(A, t, Z);
+        lds_pass<WT, OP, 32>(A, t, Z); lds_pass<WT, OP, 32>(A, t, Y); lds_pass<WT, OP, 32>(A, t, X);
+    } else {  // the inverse's passes in reverse order, each transposed
+        lds_pass<WT, OP, 32>(A, t, X); lds_pass<WT, OP, 32>(A, t, Y); lds_pass<WT, OP, 32>(A, t, Z);
+        lds_pass<WT, OP, 16>(A, t, Z); lds_pass<WT, OP, 16>(A, t, X); lds_pass<WT, OP, 16>(A, t, Y);
+        lds_pass<WT, OP, 8>(A, t, Y); lds_pass<WT, OP, 8>(A, t, Z); lds_pass<WT, OP, 8>(A, t, X);
+    }
+#pragma unroll 8
+    for (int k = 0; k < 32768 / TT; ++k) {
+        const int e = t + TT * k;
+        g[e] = A[(e >> 10) * TSP + ((e >> 5) & 31) * TRP + (e & 31)];
+    }
+}
+
+template <int WT>
+static void transform_lds_f32(float *blocks, int nblocks, int op, hipStream_t s)
+{
+    const dim3 grid(nblocks), blk(TT);
+    if (op == OP_FWD) hipLaunchKernelGGL((k_transform_lds_f32<WT, OP_FWD>), grid, blk, 0, s, blocks);
+    if (op == OP_INV) hipLaunchKernelGGL((k_transform_lds_f32<WT, OP_INV>), grid, blk, 0, s, blocks);
+    if (op == OP_INV_ADJ) hipLaunchKernelGGL((k_transform_lds_f32<WT, OP_INV_ADJ>), grid, blk, 0, s, blocks);
+}
+
 template <typename R, int WT, int OP>
 static void pass(R *blocks, int nblocks, int n, int ax, hipStream_t s)
 {
@@ -61,6 +143,10 @@ enum { AX_Z = 0, AX_Y = 1, AX_X = 2 };
 template <typename R, int WT>
 static void transform(R *blocks, int nblocks, int op, hipStream_t s)
 {
+    if constexpr (sizeof(R) == 4) {
+        transform_lds_f32<WT>(blocks, nblocks, op, s);
+        return;
+    }
     if (op == OP_FWD) {
         const int sched[3][3] = {{AX_X, AX_Y, AX_Z}, {AX_Z, AX_X, AX_Y}, {AX_Y, AX_Z, AX_X}};
         for (int l = 0; l < 3; ++l)
