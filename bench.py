@@ -116,8 +116,10 @@ def config0_leg(a, torch, czw, fields):
     fh, dh = f.cpu().numpy().astype(np.float64), dec.cpu().numpy()
     e = dh.astype(np.float64) - fh
     mse = float(np.mean(e ** 2))
+    import hashlib
+
     return {"workload": "128^3 f32 bumps field (seed 0), wavelet type 1, eps 1e-3, shuffle, zlib",
-            "file_bytes": file_bytes, "ratio": round(fh.size * 4 / file_bytes, 3),
+            "field_sha256": hashlib.sha256(fh.astype(np.float32).tobytes()).hexdigest(), "file_bytes": file_bytes, "ratio": round(fh.size * 4 / file_bytes, 3),
             "psnr_db": round(20 * float(np.log10((fh.max() - fh.min()) / (2 * np.sqrt(mse)))), 4),
             "linf": float(np.abs(e).max()), "_host": (fh.astype(np.float32), r, o, dh)}
 
@@ -606,6 +608,9 @@ def main():
     recs = records[: int(off[-1])].cpu().numpy()
     outh = out.cpu().numpy()
     fh = field.cpu().numpy()
+    import hashlib
+
+    field_sha256 = hashlib.sha256(fh.tobytes()).hexdigest()  # the synthetic input, for reproduction
     err_inf = float(np.max(np.abs(outh.astype(np.float64) - fh)))
     mse = float(np.mean((outh.astype(np.float64) - fh) ** 2))
     psnr = 20 * np.log10((fh.max() - fh.min()) / (2 * np.sqrt(mse))) if mse > 0 else float("inf")
@@ -695,7 +700,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plain compress+decompress, {n}^3 f32 bubble field ({a.bubbles} bubbles), "
                                f"{nblocks} blocks of 32^3",
-                   "eps": a.eps, "wtype": a.wtype, "zerobits": a.zerobits, "shuffle": a.shuffle},
+                   "eps": a.eps, "wtype": a.wtype, "zerobits": a.zerobits, "shuffle": a.shuffle,
+                   "field_sha256": field_sha256},
         "stages_ms": {"compress": round(t_c * 1e3, 4), "pack": round(t_p * 1e3, 4), "decompress": round(t_d * 1e3, 4)},
         "launch": "one HIP graph per step" if graph is not None else "three launches per step",
         "graph_matches_eager": graph_ok,
