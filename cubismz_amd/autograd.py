@@ -138,13 +138,14 @@ class MaskedIWT:
                 def forward(ctx, c, mask, wtype):
                     ctx.save_for_backward(mask)
                     ctx.wtype = int(wtype)
-                    return czw.transform_blocks(torch.where(mask, c, torch.zeros_like(c)).contiguous(), wtype, "iwt")
+                    # where() makes a fresh contiguous tensor, transformed in place
+                    return czw.transform_blocks(torch.where(mask, c, 0.0), wtype, "iwt")
 
                 @staticmethod
                 def backward(ctx, g):
                     (mask,) = ctx.saved_tensors
                     ga = czw.transform_blocks(g.contiguous().clone(), ctx.wtype, "iwt_adjoint")
-                    return torch.where(mask, ga, torch.zeros_like(ga)), None, None
+                    return torch.where(mask, ga, 0.0), None, None
 
             cls._fn = _F
         return cls._fn
