@@ -119,3 +119,34 @@ def test_batch_corrupt_record_sets_error(torch_gpu):
     bt.records[o:o + 4] = torch.tensor([255, 255, 255, 127], dtype=torch.uint8, device="cuda")
     with pytest.raises(czw.CZWError):
         bt.decompress()
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_random_batches(torch_gpu, case):
+    """Seeded random batches: 1..12 fields of random shapes and eps, one
+    precision, random wavelet type, zerobits, shuffle and launch shape; each
+    field equals its per-field compress and decode byte for byte."""
+    torch = torch_gpu
+    rng = np.random.default_rng(500 + case)
+    nf = int(rng.integers(1, 13))
+    dtype = torch.float64 if rng.random() < 0.3 else torch.float32
+    shapes = [tuple(int(32 * rng.integers(1, 4)) for _ in range(3)) for _ in range(nf)]
+    eps = [float(10 ** rng.uniform(-5, -2)) for _ in range(nf)]
+    wtype, zerobits, shuffle = int(rng.integers(1, 4)), int(rng.choice([0, 4])), bool(rng.random() < 0.5)
+    groups, queue = int(rng.choice([0, 1, 7])), int(rng.integers(0, 2))
+    print(f"[case] random batch {case}: {nf} fields {dtype} wtype={wtype} zb={zerobits} sh={shuffle} "
+          f"groups={groups} queue={queue}", flush=True)
+    fs = [fields.bubble_field(s, seed=900 + 31 * case + k, nbubbles=int(rng.integers(1, 8)), dtype=dtype,
+                              device="cuda") for k, s in enumerate(shapes)]
+    bt = B.Batch(fs, eps, wtype=wtype, zerobits=zerobits, shuffle=shuffle)
+    bt.set_launch(groups, queue)
+    bt.compress()
+    bt.decompress(check=True)
+    torch.cuda.synchronize()
+    iv = torch.int32 if dtype == torch.float32 else torch.int64
+    for k, f in enumerate(fs):
+        rec, off = czw.compress_field(f, eps[k], wtype, zerobits, shuffle)
+        brec, boff = bt.field_records(k)
+        assert torch.equal(boff, off) and torch.equal(brec, rec[: int(off[-1])]), k
+        dec = czw.decompress_field(rec, off, f.shape, dtype, wtype, shuffle)
+        assert torch.equal(bt.outputs[k].view(iv), dec.view(iv)), k
