@@ -372,3 +372,57 @@ def test_multi_subdomain_writer_rejects_bad_split(host, tmp_path):
     for sub in ((3, 1, 1), (0, 1, 1), (1, 1, 2)):
         with pytest.raises(czfile.CzFileError):
             czfile.write(str(tmp_path / "bad.cz"), rec, off, (2, 2, 1), 4, -1, 0.0, sub_bpd=sub)
+
+
+@pytest.mark.parametrize("case", range(24))
+def test_random_container_roundtrip(host, tmp_path, case):
+    """Seeded random files: grid and subdomain split, precision, encoder and
+    record sizes (payload 4096 + s*k bytes of random or repetitive data);
+    the reader returns every block's record (its stream is in file order,
+    subdomain by subdomain, and offsets[b] finds block b in it)."""
+    rng = np.random.default_rng(7000 + case)
+    prec = int(rng.choice([4, 8]))
+    sub = tuple(int(v) for v in rng.integers(1, 4, 3))
+    bpd = tuple(s * int(k) for s, k in zip(sub, rng.integers(1, 3, 3)))
+    nb = int(np.prod(bpd))
+    enc = str(rng.choice(["zlib", "lz4", "none"]))
+    sizes = 4096 + prec * rng.integers(0, 300, nb)
+    sizes[rng.random(nb) < 0.1] = 4096 + prec * 32768  # some full blocks
+    off = np.zeros(nb + 1, np.uint64)
+    off[1:] = np.cumsum(sizes + 4)
+    rec = np.empty(int(off[-1]), np.uint8)
+    for b in range(nb):
+        o = int(off[b])
+        rec[o:o + 4] = np.frombuffer(np.int32(sizes[b]).tobytes(), np.uint8)
+        body = rng.integers(0, 256, int(sizes[b]), dtype=np.uint8) if rng.random() < 0.5 else \
+            np.resize(rng.integers(0, 4, 97, dtype=np.uint8), int(sizes[b]))
+        rec[o + 4:o + 4 + int(sizes[b])] = body
+    p = tmp_path / "r.cz"
+    czfile.write(str(p), rec, off, bpd, prec, int(rng.integers(1, 4)), 1e-3, encoder=enc, sub_bpd=sub)
+    with czfile.CzFile(str(p)) as f:
+        assert f.bpd == bpd and f.sub_bpd == sub and f.prec == prec and f.encoder == enc
+        r2, o2 = f.records()
+    assert len(r2) == len(rec) and int(o2[-1]) == len(rec)
+    for b in range(nb):
+        n = int(sizes[b]) + 4
+        assert np.array_equal(r2[int(o2[b]):int(o2[b]) + n], rec[int(off[b]):int(off[b]) + n]), b
+    if sub == bpd:  # one subdomain: file order is block order
+        assert np.array_equal(o2, off) and np.array_equal(r2, rec)
+
+
+@pytest.mark.parametrize("case", range(16))
+def test_random_lz4_roundtrip(host, case):
+    """LZ4 stage on seeded random inputs (lengths across the 16/32-bit hash
+    switch, random, repetitive and mixed content): decode(encode(x)) == x."""
+    rng = np.random.default_rng(8000 + case)
+    n = int(rng.choice([0, 1, 12, 13, 300, 65546, 65547, 65548, 200000]))
+    kind = case % 3
+    if kind == 0:
+        x = rng.integers(0, 256, n, dtype=np.uint8)
+    elif kind == 1:
+        x = np.resize(rng.integers(0, 256, int(rng.integers(1, 64)), dtype=np.uint8), n)
+    else:
+        x = rng.integers(0, 256, n, dtype=np.uint8)
+        x[: n // 2] = 0
+    data = x.tobytes()
+    assert czfile.lz4_decompress(czfile.lz4_compress(data), len(data)) == data
