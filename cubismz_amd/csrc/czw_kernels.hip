@@ -426,11 +426,13 @@ __global__ __launch_bounds__(1024) void k_scan_records(const uint32_t *__restric
     const int t = threadIdx.x;
     uint64_t carry = 0;
     for (int c0 = 0; c0 < n; c0 += CH) {
+        // unconditional loads at an index clamped into [0, n), then a select,
+        // so that all of a thread's loads are in flight at once
+        uint32_t q[CH / 1024];
 #pragma unroll
-        for (int j = 0; j < CH / 1024; ++j) {
-            const int i = c0 + t + 1024 * j;
-            P[t + 1024 * j] = i < n ? nbytes[i] + 4u : 0u;
-        }
+        for (int j = 0; j < CH / 1024; ++j) q[j] = nbytes[min(c0 + t + 1024 * j, n - 1)];
+#pragma unroll
+        for (int j = 0; j < CH / 1024; ++j) P[t + 1024 * j] = c0 + t + 1024 * j < n ? q[j] + 4u : 0u;
         __syncthreads();
         uint32_t v[CH / 1024], sum = 0;
 #pragma unroll
@@ -483,15 +485,36 @@ __global__ __launch_bounds__(PK_T) void k_pack_records(const uint8_t *__restrict
     __shared__ uint64_t start[PK_B];
     const int b0 = blockIdx.x * PK_B, t = threadIdx.x;
     const int nbk = min(PK_B, nblocks - b0);
+    // the slots' loads do not depend on the offsets: issued first, they are
+    // in flight while the offsets are found
+    constexpr int MW = 1024 / PK_T;
+    uint32_t m[PK_B][MW], v[PK_B][PK_W];
+#pragma unroll
+    for (int kb = 0; kb < PK_B; ++kb) {
+        if (kb < nbk) {
+            const int b = b0 + kb;
+            const uint32_t S = (nbytes[b] - MASK_BYTES) / 4, sc = nbytes[nblocks + b];
+            const uint32_t *src = reinterpret_cast<const uint32_t *>(slots + (size_t)b * slot_bytes);
+            const uint32_t *corner = src + MASK_BYTES / 4, *l32 = corner + CORNER_MAX;
+#pragma unroll
+            for (int k = 0; k < MW; ++k) m[kb][k] = src[t + PK_T * k];
+#pragma unroll
+            for (int k = 0; k < PK_W; ++k) {
+                const uint32_t i = t + PK_T * k;
+                v[kb][k] = i < S ? (i < sc ? corner[i] : l32[i - sc]) : 0u;
+            }
+        }
+    }
     if (fused) {
         uint64_t part = 0;
         for (int c = 0; c < b0; c += 8 * PK_T) {
+            // unconditional loads at an index clamped into [0, b0), then a
+            // select: a load under a branch waits for itself before the next
             uint32_t q[8];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int i = c + t + PK_T * k;
-                q[k] = i < b0 ? nbytes[i] + 4u : 0u;
-            }
+            for (int k = 0; k < 8; ++k) q[k] = nbytes[min(c + t + PK_T * k, b0 - 1)];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) q[k] = c + t + PK_T * k < b0 ? q[k] + 4u : 0u;
 #pragma unroll
             for (int k = 0; k < 8; ++k) part += q[k];
         }
@@ -514,24 +537,6 @@ __global__ __launch_bounds__(PK_T) void k_pack_records(const uint8_t *__restrict
         start[t] = offsets[b0 + t];
     }
     __syncthreads();
-    constexpr int MW = 1024 / PK_T;
-    uint32_t m[PK_B][MW], v[PK_B][PK_W];
-#pragma unroll
-    for (int kb = 0; kb < PK_B; ++kb) {
-        if (kb < nbk) {
-            const int b = b0 + kb;
-            const uint32_t S = (nbytes[b] - MASK_BYTES) / 4, sc = nbytes[nblocks + b];
-            const uint32_t *src = reinterpret_cast<const uint32_t *>(slots + (size_t)b * slot_bytes);
-            const uint32_t *corner = src + MASK_BYTES / 4, *l32 = corner + CORNER_MAX;
-#pragma unroll
-            for (int k = 0; k < MW; ++k) m[kb][k] = src[t + PK_T * k];
-#pragma unroll
-            for (int k = 0; k < PK_W; ++k) {
-                const uint32_t i = t + PK_T * k;
-                v[kb][k] = i < S ? (i < sc ? corner[i] : l32[i - sc]) : 0u;
-            }
-        }
-    }
 #pragma unroll
     for (int kb = 0; kb < PK_B; ++kb) {
         if (kb < nbk) {
