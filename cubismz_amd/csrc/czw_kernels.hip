@@ -685,7 +685,10 @@ static int batch_compress(const czw_batch_t *B, hipStream_t s)
     const uint32_t zkeep = B->zerobits ? ~((1u << B->zerobits) - 1u) : 0xffffffffu;
     const void *fn = wt == 1 ? (const void *)k_compress_f32<1, LAYOUT_BATCH>
                    : wt == 2 ? (const void *)k_compress_f32<2, LAYOUT_BATCH> : (const void *)k_compress_f32<3, LAYOUT_BATCH>;
-    const int groups = B->groups ? B->groups : resident_groups(fn, CT);
+    // at most the resident groups: a groups value tuned for the decoder (two
+    // per CU) would otherwise queue a second persistent wave behind the first
+    const int R = resident_groups(fn, CT);
+    const int groups = B->groups ? std::min(B->groups, R) : R;
     const dim3 grid(std::min(B->nblocks, groups)), blk(CT);
     const float *src0 = nullptr;
     const FieldGeom g{0, 0, 0};
@@ -863,8 +866,11 @@ int czw_batch_autotune(czw_batch_t *batch, int stage, hipStream_t stream, float 
 {
     if (!batch_ok(batch) || (stage != 1 && stage != 2)) return CZW_EINVAL;
     const void *fn = (const void *)k_compress_f32<1, LAYOUT_BATCH>;
-    const int R = resident_groups(fn, CT);
-    const int shapes[4][2] = {{R, 0}, {R, 1}, {R / 2 > 0 ? R / 2 : 1, 0}, {R / 2 > 0 ? R / 2 : 1, 1}};
+    const int R = resident_groups(fn, CT);  // the compressor's: one group per CU
+    // the compressor is tried at R and R/2 groups; the decoder, which fits two
+    // groups per CU for binary32, at 2R and R
+    const int G1 = stage == 1 ? R : 2 * R, G2 = stage == 1 ? (R / 2 > 0 ? R / 2 : 1) : R;
+    const int shapes[4][2] = {{G1, 0}, {G1, 1}, {G2, 0}, {G2, 1}};
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return CZW_EHIP;
     czw_batch_t trial = *batch;
