@@ -159,7 +159,8 @@ typedef struct czw_batch {
     uint64_t *offsets;         /* nblocks + 1 */
     size_t records_bytes;      /* size of the records buffer (decompress bound) */
     uint32_t *err;             /* decompress error bits; cleared by each decompress */
-    int groups;                /* persistent workgroups; 0 = as many as are resident */
+    int groups;                /* persistent workgroups; 0 = as many as are resident (the binary32
+                                  compress stage uses at most its resident count) */
     int queue;                 /* 1: dynamic block queue; 0: static stride */
 } czw_batch_t;
 
