@@ -2,15 +2,16 @@
 """Benchmark: CubismZ wavelet compression stage on one MI355X.
 
 Workload (the metric's configuration): one 512^3 float32 cavitation-like
-pressure field (background 1.0 + 200 bubbles, cubismz_amd.fields.bubble_field,
-seed 1), 16^3 blocks of 32^3, eps = 1e-3, the fourth-order interpolating
+pressure field (background 1.0 + 200 bubbles with a plain 1/r pressure halo,
+cubismz_amd.fields.bubble_field, seed 1), 16^3 blocks of 32^3, eps = 1e-3,
+the fourth-order interpolating
 wavelet (wavelet type 1, WaveletsOnInterval.h:33-51, 97-153), no zerobits, no
 shuffle.  The reference CLI's default, the third-order average-interpolating
 wavelet (type 3, Tools/Test_IO_Compressed.h:305), is timed beside it.
 
-A step = compress (fwt + threshold + mask + survivor stream + record framing)
-+ decompress (decode + iwt) of the whole field, input already resident in
-HBM: three launches (--graph: one HIP graph replay).  value = field bytes per
+A step = compress (fwt + threshold + mask + survivor stream, then the record
+framing kernel) + decompress (decode + iwt) of the whole field, input already
+resident in HBM: three launches (--graph: one HIP graph replay).  value = field bytes per
 step / step time, in GB/s, over --steps timed steps after --warmup untimed
 ones; before those, untimed steps run for --settle-ms so that the clocks have
 ramped up from idle (a few milliseconds of work after idle run measurably
@@ -23,7 +24,8 @@ Also reported:
   cpu_baseline  the reference's own compressor and decoder built from its
                 sources into oracle/_ref (the oracle, a C restatement of it,
                 where that build is absent), OpenMP over blocks, on the same
-                field on this host's cores; also on one thread.
+                field on every core this process may run on (the count
+                and the host's total are reported); also on one thread.
   parity        GPU records and decoded field compared bit for bit with the
                 oracle for every block.
 """
@@ -155,7 +157,7 @@ def batch_leg(a, torch, czw, fields):
     from cubismz_amd import batch as B
 
     shapes, eps = fields.batched_shapes(64, seed=7)
-    fs = [fields.bubble_field(s, seed=100 + k, nbubbles=24, dtype=torch.float32, device="cuda")
+    fs = [fields.bubble_field(s, seed=100 + k, nbubbles=fields.bubbles_for(s), dtype=torch.float32, device="cuda")
           for k, s in enumerate(shapes)]
     total_bytes = sum(f.numel() * 4 for f in fs)
     bt = B.Batch(fs, list(eps), wtype=a.wtype)
@@ -172,7 +174,7 @@ def batch_leg(a, torch, czw, fields):
     def plain():
         for k, f in enumerate(fs):
             rec, off, ws, out = bufs[k]
-            czw.compress_field(f, float(eps[k]), a.wtype, 0, False, rec, off, ws)
+            czw.compress_field(f, float(eps[k]), a.wtype, 0, False, rec, off, ws, check=False)
             czw.decompress_field(rec, off, f.shape, torch.float32, a.wtype, False, out=out, err=err, check=False)
 
     reps = max(3, a.steps // 4)
@@ -215,13 +217,15 @@ def batch_leg(a, torch, czw, fields):
         brec, boff = bt.field_records(k)
         same &= bool(torch.equal(boff, off)) and bool(torch.equal(brec, rec[: int(off[-1])]))
         same &= bool(torch.equal(bt.outputs[k].view(torch.int32), out.view(torch.int32)))
-    return {"fields": 64, "blocks": bt.nblocks, "field_bytes": total_bytes,
+    return {"workload": "64 float32 bubble fields, edges from {64..384}, 200 bubbles per 512^3 cells (at least 1), "
+                        "eps log-uniform in [1e-4, 1e-2]",
+            "fields": 64, "blocks": bt.nblocks, "field_bytes": total_bytes,
             "GBps_batched": round(total_bytes / (min(t_graph, t_eager) * 1e-3) / 1e9, 3),
             "GBps_plain_64_launches": round(total_bytes / (t_plain * 1e-3) / 1e9, 3),
             "ms_batched": round(min(t_graph, t_eager), 4), "ms_batched_graph": round(t_graph, 4),
             "ms_batched_launches": round(t_eager, 4), "ms_plain": round(t_plain, 4),
             "speedup_vs_plain": round(t_plain / min(t_graph, t_eager), 3),
-            "autotuned": {"compress_ms": round(ms_c, 4), "decompress_ms": round(ms_d, 4), **shape},
+            "autotuned": {"compress_ms": round(ms_c, 4), "decompress_ms": round(ms_d, 4), "shapes": shape},
             "matches_per_field": same}
 
 
@@ -418,24 +422,32 @@ def main():
             field.cpu().numpy().tofile(a.field_cache)
     torch.cuda.synchronize()
 
-    slot = czw.slot_bytes(prec)
-    slots = torch.empty(nblocks * slot, dtype=torch.uint8, device=dev)
-    nbytes = torch.empty(2 * nblocks, dtype=torch.int32, device=dev)
     records = torch.empty(czw.records_bound(nblocks, prec), dtype=torch.uint8, device=dev)
     offsets = torch.empty(nblocks + 1, dtype=torch.int64, device=dev)
+    workspace = torch.empty(czw.workspace_bytes(nblocks, prec), dtype=torch.uint8, device=dev)
+    # the same workspace seen as the two-stage path's slots and size words
+    slot = czw.slot_bytes(prec)
+    slots = workspace[256:256 + nblocks * slot]
+    nbytes = workspace[256 + nblocks * slot:256 + nblocks * slot + 8 * nblocks].view(torch.int32)
     out = torch.empty_like(field)
     err = torch.zeros(1, dtype=torch.int32, device=dev)
 
-    def step(ev=None):
+    def step(ev=None, wtype=None, eps=None, zerobits=None):
+        """compress (czw_compress_field: compressor + record framing) + decompress;
+        with events, the two compress stages are launched through their own
+        entry points (the same kernels) so that each is timed alone"""
+        wt = a.wtype if wtype is None else wtype
+        e_ = a.eps if eps is None else eps
+        zb = a.zerobits if zerobits is None else zerobits
         if ev:
             ev[0].record()
-        czw.compress_field_slots(field, a.eps, a.wtype, a.zerobits, slots, nbytes)
-        if ev:
+            czw.compress_field_slots(field, e_, wt, zb, slots, nbytes)
             ev[1].record()
-        czw.pack_records(slots, slot, nbytes, nblocks, a.shuffle, records, offsets)
-        if ev:
+            czw.pack_records(slots, slot, nbytes, nblocks, bool(a.shuffle), records, offsets)
             ev[2].record()
-        czw.decompress_field(records, offsets, shape, torch.float32, a.wtype, a.shuffle, out=out, err=err,
+        else:
+            czw.compress_field(field, e_, wt, zb, bool(a.shuffle), records, offsets, workspace, check=False)
+        czw.decompress_field(records, offsets, shape, torch.float32, wt, a.shuffle, out=out, err=err,
                              check=False)
         if ev:
             ev[3].record()
@@ -456,10 +468,10 @@ def main():
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()
-    if int(err.item()):
-        raise SystemExit("decoder reported corrupt records")
+    if int(err.item()) or czw.workspace_errors(workspace):
+        raise SystemExit("decoder or compressor reported an error")
 
-    # Optionally one HIP graph per step (compress, pack, decompress).  The
+    # Optionally one HIP graph per step (fill, compress, decompress).  The
     # three launches take about 40 us of host time per step of about 330 us
     # of GPU time, so the GPU does not wait for the host either way, and the
     # replays measured about 2% slower than plain launches.  The kernels
@@ -524,15 +536,11 @@ def main():
         if w == a.wtype:
             continue
         for _ in range(a.warmup):
-            czw.compress_field_slots(field, a.eps, w, a.zerobits, slots, nbytes)
-            czw.pack_records(slots, slot, nbytes, nblocks, a.shuffle, records, offsets)
-            czw.decompress_field(records, offsets, shape, torch.float32, w, a.shuffle, out=out, err=err, check=False)
+            step(wtype=w)
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(a.steps):
-            czw.compress_field_slots(field, a.eps, w, a.zerobits, slots, nbytes)
-            czw.pack_records(slots, slot, nbytes, nblocks, a.shuffle, records, offsets)
-            czw.decompress_field(records, offsets, shape, torch.float32, w, a.shuffle, out=out, err=err, check=False)
+            step(wtype=w)
         e1.record()
         torch.cuda.synchronize()
         tw = e0.elapsed_time(e1) / a.steps * 1e-3
@@ -545,10 +553,7 @@ def main():
         settings = [(eps, zb) for eps in (1e-2, 1e-3, 1e-4) for zb in (0, 4)]
 
         def sw(eps, zb):
-            czw.compress_field_slots(field, eps, a.wtype, zb, slots, nbytes)
-            czw.pack_records(slots, slot, nbytes, nblocks, a.shuffle, records, offsets)
-            czw.decompress_field(records, offsets, shape, torch.float32, a.wtype, a.shuffle, out=out, err=err,
-                                 check=False)
+            step(eps=eps, zerobits=zb)
 
         # timed back to back, so that the GPU never idles between settings
         # (clocks drop within milliseconds of idle); the best of three windows,
@@ -590,15 +595,21 @@ def main():
     # restore the headline wavelet's records for the checks below
     step()
     torch.cuda.synchronize()
+    if czw.workspace_errors(workspace):
+        raise SystemExit("compressor reported an error")
     off = offsets.cpu().numpy()
-    nb = nbytes[:nblocks].cpu().numpy().astype(np.int64)
+    rec_total = int(off[-1])
+    nb = np.diff(off).astype(np.int64) - 4  # payload bytes per block
     payload = int(nb.sum())
     surv = int(((nb - 4096) // prec).sum())
     rho = surv / field.numel()
-    # algorithmic bytes (per launch): compress reads the field and writes
-    # 4 + 4096 + s*S_b per block; decompress reads that and writes the field
+    # algorithmic bytes per launch: the compressor reads the field and writes
+    # the payloads (4096 + s*S_b per block; framing them into records is the
+    # separate pack kernel's work); the decoder reads the records (4 + 4096 +
+    # s*S_b per block) and their offsets and writes the field
     alg_c = field_bytes + payload
-    alg_d = field_bytes + payload + 4 * nblocks
+    alg_d = field_bytes + rec_total + 8 * (nblocks + 1)
+    alg_step = alg_c + alg_d  # the step's algorithmic bytes: framing adds none
     kern = {"k_compress_f32": (alg_c, t_c), "k_decompress_h<float>": (alg_d, t_d)}
     dom = max(kern, key=lambda k: kern[k][1])
     achieved = kern[dom][0] / kern[dom][1]
@@ -621,7 +632,8 @@ def main():
     if not a.no_cpu:
         from oracle import oracle as O
 
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        affinity = len(os.sched_getaffinity(0))
+        threads = a.cpu_threads or affinity
         threads = O.set_threads(threads)
         print(f"[bench] cpu baseline ({threads} threads) + parity over {nblocks} blocks", file=sys.stderr,
               flush=True)
@@ -631,6 +643,7 @@ def main():
         ofield = O.decompress_field(oslots, onb, shape, a.wtype, "f32", a.shuffle)
         tc2 = time.perf_counter()
         cpu = {"value": round(field_bytes / (tc2 - tc0) / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+               "host_cpus": os.cpu_count(), "affinity_cpus": affinity,
                "sample": f"oracle over the whole {n}^3 field ({nblocks} blocks), compress {tc1 - tc0:.3f} s + "
                          f"decompress {tc2 - tc1:.3f} s"}
         if O.ref_available("f32", a.zerobits, bool(a.shuffle)):
@@ -645,7 +658,7 @@ def main():
                 tr2 = time.perf_counter()
                 t_rc, t_rd = t_rc + (tr1 - tr0), t_rd + (tr2 - tr1)
             cpu = {"value": round(reps * field_bytes / (t_rc + t_rd) / 1e9, 4), "unit": "GB/s", "cores": threads,
-                   "kind": "reference",
+                   "kind": "reference", "host_cpus": os.cpu_count(), "affinity_cpus": affinity,
                    "sample": f"whole {n}^3 field ({nblocks} blocks) {reps} times, compress {t_rc / reps:.3f} s + "
                              f"decompress {t_rd / reps:.3f} s per pass; the oracle port on the same: "
                              f"{field_bytes / (tc2 - tc0) / 1e9:.3f} GB/s",
@@ -698,17 +711,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"plain compress+decompress, {n}^3 f32 bubble field ({a.bubbles} bubbles), "
+        "config": {"workload": f"plain compress+decompress, {n}^3 f32 bubble field ({a.bubbles} bubbles, "
+                               f"1/r halo), "
                                f"{nblocks} blocks of 32^3",
                    "eps": a.eps, "wtype": a.wtype, "zerobits": a.zerobits, "shuffle": a.shuffle,
                    "field_sha256": field_sha256},
         "stages_ms": {"compress": round(t_c * 1e3, 4), "pack": round(t_p * 1e3, 4), "decompress": round(t_d * 1e3, 4)},
-        "launch": "one HIP graph per step" if graph is not None else "three launches per step",
+        "launch": "one HIP graph per step" if graph is not None else "three launches per step (compressor, record framing, decoder)",
         "graph_matches_eager": graph_ok,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved / 1e9, 2), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic,
                      "alg_bytes_per_launch": kern[dom][0], "avg_launch_ms": round(kern[dom][1] * 1e3, 4)},
-        "step_roofline_frac": round((alg_c + alg_d) / t_step / HBM_PEAK, 4),
+        "step_roofline_frac": round(alg_step / t_step / HBM_PEAK, 4),
         "other_wtypes": other,
         "eps_zerobits_sweep": sweep or None,
         "config0": {k: v for k, v in config0.items() if k != "_host"} if config0 else None,

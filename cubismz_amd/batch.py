@@ -20,11 +20,17 @@ FIELD_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("nx", "<i4"), ("ny", "<
 MAX_FIELDS = 64
 
 
+class _Shape(C.Structure):
+    _fields_ = [("groups", C.c_int), ("queue", C.c_int)]
+
+
 class _Batch(C.Structure):
     _fields_ = [("fields", C.c_void_p), ("nfields", C.c_int), ("nblocks", C.c_int), ("prec", C.c_int),
                 ("wtype", C.c_int), ("zerobits", C.c_int), ("shuffle", C.c_int), ("workspace", C.c_void_p),
                 ("records", C.c_void_p), ("offsets", C.c_void_p), ("records_bytes", C.c_size_t),
-                ("err", C.c_void_p), ("groups", C.c_int), ("queue", C.c_int)]
+                ("err", C.c_void_p), ("compress", _Shape), ("decompress", _Shape)]
+
+STAGES = ("compress", "decompress")
 
 
 def _lib():
@@ -126,18 +132,28 @@ class Batch:
         self.err = torch.zeros(1, dtype=torch.int32, device=dev)
         self.desc = _Batch(self.table_dev.data_ptr(), len(self.fields), self.nblocks, self.prec, int(wtype),
                            int(zerobits), int(bool(shuffle)), self.workspace.data_ptr(), self.records.data_ptr(),
-                           self.offsets.data_ptr(), self.records.numel(), self.err.data_ptr(), 0, 0)
+                           self.offsets.data_ptr(), self.records.numel(), self.err.data_ptr(), _Shape(0, 0),
+                           _Shape(0, 0))
 
     @property
     def launch_shape(self):
-        return {"groups": self.desc.groups, "queue": self.desc.queue}
+        """Launch shape of each stage: {"compress": {"groups", "queue"}, "decompress": {...}}."""
+        return {st: {"groups": getattr(self.desc, st).groups, "queue": getattr(self.desc, st).queue}
+                for st in STAGES}
 
-    def set_launch(self, groups=0, queue=0):
-        self.desc.groups, self.desc.queue = int(groups), int(queue)
+    def set_launch(self, groups=0, queue=0, stage=None):
+        """Set the launch shape of one stage, or of both when stage is None."""
+        for st in (STAGES if stage is None else (stage,)):
+            sh = getattr(self.desc, st)
+            sh.groups, sh.queue = int(groups), int(queue)
 
-    def compress(self):
+    def compress(self, check=False):
+        """Compress every field into ``records``/``offsets``.  With check, wait
+        for the device and raise on a compressor error (binary32 look-back)."""
         torch = czw._torch()
         czw._check(_lib().czw_batch_compress(C.byref(self.desc), czw._stream(torch)), "czw_batch_compress")
+        if check:
+            czw.check_workspace(self.workspace)
 
     def decompress(self, check=True):
         torch = czw._torch()

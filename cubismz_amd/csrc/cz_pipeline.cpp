@@ -150,7 +150,8 @@ extern "C" int czh_compress_device_field(const char *path, const void *field, in
         (void)hipEventRecord(ev_c1, stream);
         uint64_t *oh = reinterpret_cast<uint64_t *>(off_h[k & 1].p);
         if (hipMemcpyAsync(oh, d.offsets, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, stream) != hipSuccess ||
-            hipEventRecord(ev_off, stream) != hipSuccess || hipEventSynchronize(ev_off) != hipSuccess)
+            hipEventRecord(ev_off, stream) != hipSuccess || hipEventSynchronize(ev_off) != hipSuccess ||
+            czw_compress_status(d.workspace, stream) != CZW_OK)
             return CZH_EIO;
         float ms = 0;
         if (hipEventElapsedTime(&ms, ev_c0, ev_c1) == hipSuccess) t_gpu += ms * 1e-3;
@@ -200,7 +201,7 @@ extern "C" int czh_compress_device_field(const char *path, const void *field, in
                 uint64_t *oh = reinterpret_cast<uint64_t *>(off_h[(k + 1) & 1].p);
                 if (hipMemcpyAsync(oh, d.offsets, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, stream) !=
                         hipSuccess ||
-                    hipStreamSynchronize(stream) != hipSuccess) {
+                    hipStreamSynchronize(stream) != hipSuccess || czw_compress_status(d.workspace, stream) != CZW_OK) {
                     rc = CZH_EIO;
                     break;
                 }

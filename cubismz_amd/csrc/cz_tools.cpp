@@ -347,7 +347,7 @@ int tool_main(int argc, char **argv)
                                           shuffle, drec, doff, dws, nullptr);
         std::vector<uint64_t> off(nb + 1);
         std::vector<uint8_t> rec;
-        if (rc == CZW_OK) {
+        if (rc == CZW_OK && czw_compress_status(dws, nullptr) == CZW_OK) {
             HIPCHECK(hipMemcpy(off.data(), doff, off.size() * sizeof(uint64_t), hipMemcpyDeviceToHost));
             rec.resize(off[nb]);
             HIPCHECK(hipMemcpy(rec.data(), drec, off[nb], hipMemcpyDeviceToHost));
@@ -357,7 +357,8 @@ int tool_main(int argc, char **argv)
         (void)hipFree(dws);
         (void)hipFree(doff);
         std::vector<uint8_t> file;
-        if (rc != CZW_OK || cz::build_cz(p, rec.data(), off.data(), nb, 0, file) || !cz::write_file(path, file)) {
+        if (rc != CZW_OK || rec.size() != off[nb] || off[nb] == 0 || cz::build_cz(p, rec.data(), off.data(), nb, 0, file) ||
+            !cz::write_file(path, file)) {
             std::fprintf(stderr, "hdf2cz: compressing to %s failed\n", path.c_str());
             return 1;
         }

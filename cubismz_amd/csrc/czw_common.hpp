@@ -341,5 +341,8 @@ int compress_slots_f64(int layout, const void *src, int nblocks, FieldGeom g, fl
                        uint32_t *queue, hipStream_t s);
 int pack_records_f64(const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle, uint8_t *records,
                      uint64_t *offsets, hipStream_t s);
+// Resident workgroups (device-wide) of the batch-layout half-block kernels.
+int resident_compress_f64(int layout, int wtype);
+int resident_decompress(int prec, int layout, int wtype, int shuffle);
 
 }  // namespace czw

@@ -690,7 +690,10 @@ static int launch_compress_h(int wt, const R *src, FieldGeom g, int nblocks, flo
 {
     const void *fn = wt == 1 ? (const void *)k_compress_h<R, 1, LAYOUT>
                    : wt == 2 ? (const void *)k_compress_h<R, 2, LAYOUT> : (const void *)k_compress_h<R, 3, LAYOUT>;
-    const dim3 grid(std::min(nblocks, groups ? groups : resident_groups(fn, NT))), blk(NT);
+    // never more than are resident: a second persistent wave would start only
+    // after the first had drained the queue
+    const int res = resident_groups(fn, NT);
+    const dim3 grid(std::min(nblocks, groups ? std::min(groups, res) : res)), blk(NT);
     if (wt == 1) hipLaunchKernelGGL((k_compress_h<R, 1, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, cnt, fields, nf, queue);
     if (wt == 2) hipLaunchKernelGGL((k_compress_h<R, 2, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, cnt, fields, nf, queue);
     if (wt == 3) hipLaunchKernelGGL((k_compress_h<R, 3, LAYOUT>), grid, blk, 0, s, src, g, nblocks, eps, zkeep, slots, nb, cnt, fields, nf, queue);
@@ -752,7 +755,8 @@ static int launch_decompress_hs(int wt, const uint8_t *rec, uint64_t rb, const u
     const void *fn = wt == 1 ? (const void *)k_decompress_h<R, 1, LAYOUT, SHUF>
                    : wt == 2 ? (const void *)k_decompress_h<R, 2, LAYOUT, SHUF>
                              : (const void *)k_decompress_h<R, 3, LAYOUT, SHUF>;
-    const dim3 grid(std::min(nblocks, groups ? groups : resident_groups(fn, NT))), blk(NT);
+    const int res = resident_groups(fn, NT);
+    const dim3 grid(std::min(nblocks, groups ? std::min(groups, res) : res)), blk(NT);
     if (wt == 1) hipLaunchKernelGGL((k_decompress_h<R, 1, LAYOUT, SHUF>), grid, blk, 0, s, rec, rb, offsets, nblocks, dst, g, err, fields, nf, queue);
     if (wt == 2) hipLaunchKernelGGL((k_decompress_h<R, 2, LAYOUT, SHUF>), grid, blk, 0, s, rec, rb, offsets, nblocks, dst, g, err, fields, nf, queue);
     if (wt == 3) hipLaunchKernelGGL((k_decompress_h<R, 3, LAYOUT, SHUF>), grid, blk, 0, s, rec, rb, offsets, nblocks, dst, g, err, fields, nf, queue);
@@ -808,6 +812,35 @@ int decompress_half(int prec, int layout, const void *records, size_t rec_bytes,
         return decompress_t<float>(layout, rec, (uint64_t)rec_bytes, offsets, nblocks, g, wt, shuffle, dst, err,
                                    fields, nf, groups, queue, s);
     return CZW_EINVAL;
+}
+
+
+// Resident workgroups of the batch-layout kernels a launch would use.
+int resident_compress_f64(int layout, int wtype)
+{
+    (void)layout;  // the batch table is the only layout asked about
+    const int wt = wt_index(wtype);
+    const void *fn = wt == 1 ? (const void *)k_compress_h<double, 1, LAYOUT_BATCH>
+                   : wt == 2 ? (const void *)k_compress_h<double, 2, LAYOUT_BATCH>
+                             : (const void *)k_compress_h<double, 3, LAYOUT_BATCH>;
+    return resident_groups(fn, NT);
+}
+
+template <typename R, bool SHUF>
+static int resident_dec(int wt)
+{
+    const void *fn = wt == 1 ? (const void *)k_decompress_h<R, 1, LAYOUT_BATCH, SHUF>
+                   : wt == 2 ? (const void *)k_decompress_h<R, 2, LAYOUT_BATCH, SHUF>
+                             : (const void *)k_decompress_h<R, 3, LAYOUT_BATCH, SHUF>;
+    return resident_groups(fn, NT);
+}
+
+int resident_decompress(int prec, int layout, int wtype, int shuffle)
+{
+    (void)layout;
+    const int wt = wt_index(wtype);
+    if (prec == 8) return shuffle ? resident_dec<double, true>(wt) : resident_dec<double, false>(wt);
+    return shuffle ? resident_dec<float, true>(wt) : resident_dec<float, false>(wt);
 }
 
 }  // namespace czw

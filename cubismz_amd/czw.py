@@ -27,7 +27,10 @@ WAVELET_NAMES = {
     3: "AverageInterpWavelet3rdOrder",
 }
 
-_STATUS = {0: "ok", -1: "invalid argument", -2: "HIP launch failure", -3: "unsupported combination"}
+_STATUS = {0: "ok", -1: "invalid argument", -2: "HIP launch failure", -3: "unsupported combination",
+           -4: "device error (see the error bits)"}
+ERR_LOOKBACK = 4  # CZW_ERR_LOOKBACK
+OFFSET_INVALID = (1 << 64) - 1  # CZW_OFFSET_INVALID
 
 
 class CZWError(RuntimeError):
@@ -69,6 +72,12 @@ def load_library():
     L.czw_compress_field_range.argtypes = [vp, i, i, i, i, f, i, i, i, i, i, vp, vp, vp, vp]
     L.czw_transform_blocks.restype = i
     L.czw_transform_blocks.argtypes = [vp, i, i, i, i, vp]
+    L.czw_compress_lowmem_bytes.restype = sz
+    L.czw_compress_lowmem_bytes.argtypes = [i, i]
+    L.czw_compress_field_lowmem.restype = i
+    L.czw_compress_field_lowmem.argtypes = [vp, i, i, i, i, f, i, i, i, vp, vp, vp, vp]
+    L.czw_set_lookback_limit.restype = C.c_uint32
+    L.czw_set_lookback_limit.argtypes = [C.c_uint32]
     _lib = L
     return L
 
@@ -121,8 +130,26 @@ def _field_arg(field, torch):
 
 
 def size_words(prec):
-    """uint32 words per block in a compressor's size array (nbytes)."""
-    return (workspace_bytes(1, prec) - slot_bytes(prec)) // 4
+    """uint32 words per block in a compressor's size array (nbytes, see czw.h)."""
+    return {4: 2, 8: 9}[int(prec)]
+
+
+def workspace_errors(workspace):
+    """Device error bits (CZW_ERR_*) of the last compress on this workspace
+    (its header's word 1); synchronizes with the device."""
+    return int(workspace[4:8].view(_torch().int32).item())
+
+
+def check_workspace(workspace):
+    e = workspace_errors(workspace)
+    if e:
+        raise CZWError(f"compressor error bits {e:#x}" + (" (look-back gave up)" if e & ERR_LOOKBACK else ""))
+
+
+def set_lookback_limit(rereads):
+    """Re-reads of a not-ready predecessor before a binary32 block gives up
+    (0 = the default); returns the previous limit."""
+    return int(load_library().czw_set_lookback_limit(int(rereads)))
 
 
 def workspace_bytes(nblocks, prec):
@@ -146,11 +173,15 @@ def _buffers(torch, device, nblocks, prec, records, offsets, workspace):
     return records, offsets, workspace
 
 
-def compress_field(field, eps, wtype=3, zerobits=0, shuffle=False, records=None, offsets=None, workspace=None):
+def compress_field(field, eps, wtype=3, zerobits=0, shuffle=False, records=None, offsets=None, workspace=None,
+                   check=True):
     """Compress an HDF5-ordered device field f[NX][NY][NZ] (every edge a multiple of 32).
 
     Returns (records, offsets): the record stream ``[int32 nbytes][payload]``
     per block (device uint8) and the nblocks+1 record offsets (device int64).
+    With check (the default) the call waits for the device and raises if the
+    compressor reported an error; check=False leaves it asynchronous (read
+    :func:`workspace_errors` later).
     """
     torch = _torch()
     L = load_library()
@@ -162,6 +193,42 @@ def compress_field(field, eps, wtype=3, zerobits=0, shuffle=False, records=None,
                               int(bool(shuffle)), C.c_void_p(records.data_ptr()), C.c_void_p(offsets.data_ptr()),
                               C.c_void_p(workspace.data_ptr()), _stream(torch))
     _check(rc, "czw_compress_field")
+    if check:
+        check_workspace(workspace)
+    return records, offsets
+
+
+def lowmem_bytes(nblocks, prec):
+    return int(load_library().czw_compress_lowmem_bytes(int(nblocks), int(prec)))
+
+
+def compress_field_lowmem(field, eps, wtype=3, zerobits=0, shuffle=False, records=None, offsets=None, scratch=None,
+                          check=True):
+    """compress_field with a small scratch (binary32): one compressor launch
+    that frames the records itself, record offsets by device look-back.
+    Same records and offsets as compress_field."""
+    torch = _torch()
+    L = load_library()
+    nx, ny, nz = _field_arg(field, torch)
+    prec = _prec(field.dtype, torch)
+    nblocks = (nx // BLOCK) * (ny // BLOCK) * (nz // BLOCK)
+    if prec != 4:
+        raise CZWError("compress_field_lowmem is binary32 only")
+    if scratch is None:
+        scratch = torch.empty(lowmem_bytes(nblocks, prec), dtype=torch.uint8, device=field.device)
+    _need(scratch, "scratch", torch, torch.uint8, lowmem_bytes(nblocks, prec), device=field.device)
+    if records is None:
+        records = torch.empty(records_bound(nblocks, prec), dtype=torch.uint8, device=field.device)
+    if offsets is None:
+        offsets = torch.empty(nblocks + 1, dtype=torch.int64, device=field.device)
+    _need(records, "records", torch, torch.uint8, records_bound(nblocks, prec), device=field.device)
+    _need(offsets, "offsets", torch, torch.int64, 8 * (nblocks + 1), device=field.device)
+    rc = L.czw_compress_field_lowmem(C.c_void_p(field.data_ptr()), nx, ny, nz, prec, float(eps), int(wtype),
+                                     int(zerobits), int(bool(shuffle)), C.c_void_p(records.data_ptr()),
+                                     C.c_void_p(offsets.data_ptr()), C.c_void_p(scratch.data_ptr()), _stream(torch))
+    _check(rc, "czw_compress_field_lowmem")
+    if check:
+        check_workspace(scratch)
     return records, offsets
 
 
@@ -179,6 +246,7 @@ def compress_field_range(field, eps, wtype, zerobits, shuffle, block0, nblocks):
                                                  C.c_void_p(offsets.data_ptr()), C.c_void_p(workspace.data_ptr()),
                                                  _stream(torch))
     _check(rc, "czw_compress_field_range")
+    check_workspace(workspace)
     return records, offsets
 
 
@@ -221,7 +289,8 @@ def slot_bytes(prec):
     return int(load_library().czw_slot_bytes(int(prec)))
 
 
-def compress_blocks(blocks, eps, wtype=3, zerobits=0, shuffle=False, records=None, offsets=None, workspace=None):
+def compress_blocks(blocks, eps, wtype=3, zerobits=0, shuffle=False, records=None, offsets=None, workspace=None,
+                    check=True):
     """Compress device blocks [nb][32][32][32] ([z][y][x]); same outputs as compress_field."""
     torch = _torch()
     L = load_library()
@@ -236,6 +305,8 @@ def compress_blocks(blocks, eps, wtype=3, zerobits=0, shuffle=False, records=Non
                                int(bool(shuffle)), C.c_void_p(records.data_ptr()), C.c_void_p(offsets.data_ptr()),
                                C.c_void_p(workspace.data_ptr()), _stream(torch))
     _check(rc, "czw_compress_blocks")
+    if check:
+        check_workspace(workspace)
     return records, offsets
 
 

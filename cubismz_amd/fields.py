@@ -12,7 +12,8 @@ HDF5-ordered arrays f[X][Y][Z] (Z fastest), the order CubismZ's hdf2cz reads
 * ``bubble_field``  -- cavitation-like pressure: background 1.0 plus spherical
   bubbles (radius U[4,40] cells, centres uniform), tanh walls 2 cells thick,
   interior pressure log-uniform in [0.01, 100], and outside each bubble a
-  1/r halo (p_b - 1) R/r screened by exp(-(r-R)/R).
+  1/r halo (p_b - 1) R/r.  ``screened=True`` multiplies the halo by
+  exp(-(r-R)/R), so that each bubble's influence stays local.
 """
 import math
 
@@ -76,7 +77,7 @@ def bubble_params(shape, seed, nbubbles):
     return cen, rad, pin
 
 
-def bubble_field(shape=(512, 512, 512), seed=1, nbubbles=200, dtype=None, device="cpu"):
+def bubble_field(shape=(512, 512, 512), seed=1, nbubbles=200, dtype=None, device="cpu", screened=False):
     torch = _torch()
     dtype = dtype or torch.float32
     NX, NY, NZ = shape
@@ -89,10 +90,18 @@ def bubble_field(shape=(512, 512, 512), seed=1, nbubbles=200, dtype=None, device
             R, dp = float(rad[k]), float(pin[k] - 1.0)
             r = torch.sqrt((X - cen[k, 0]) ** 2 + (Y - cen[k, 1]) ** 2 + (Z - cen[k, 2]) ** 2)
             w = 0.5 * (1.0 - torch.tanh((r - R) / 2.0))
-            halo = (R / torch.clamp(r, min=R)) * torch.exp(-torch.clamp(r - R, min=0.0) / R)
+            halo = R / torch.clamp(r, min=R)
+            if screened:
+                halo = halo * torch.exp(-torch.clamp(r - R, min=0.0) / R)
             f += dp * (w + (1.0 - w) * halo)
         out[x0:x0 + nx] = f.to(dtype)
     return out
+
+
+def bubbles_for(shape, per_512cube=200):
+    """Bubble count of a field at the 512^3 configuration's density (200 per
+    512^3 cells), at least one."""
+    return max(1, int(round(per_512cube * float(np.prod(shape)) / 512.0 ** 3)))
 
 
 def batched_shapes(nfields=64, seed=7):

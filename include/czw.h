@@ -59,19 +59,28 @@ enum {
     CZW_OK = 0,
     CZW_EINVAL = -1,      /* bad argument (sizes, wtype, zerobits, null) */
     CZW_EHIP = -2,        /* a HIP launch failed */
-    CZW_EUNSUPPORTED = -3 /* combination not built */
+    CZW_EUNSUPPORTED = -3, /* combination not built */
+    CZW_EDEVICE = -4       /* the device reported an error (czw_compress_status) */
 };
 
-/* Device error bits written by the decompressors into *err (OR-ed). */
+/* Device error bits (OR-ed): the decompressors write them into *err, the
+ * binary32 compressors into the 32-bit word at byte 4 of their workspace. */
 enum {
     CZW_ERR_NBYTES = 1,   /* record size/offset invalid or outside the records buffer */
-    CZW_ERR_POPCOUNT = 2  /* mask popcount differs from the survivor count */
+    CZW_ERR_POPCOUNT = 2, /* mask popcount differs from the survivor count */
+    CZW_ERR_LOOKBACK = 4  /* a block gave up waiting for its predecessors' record sizes */
 };
+
+/* offsets[] value of a block whose record was not written (CZW_ERR_LOOKBACK). */
+#define CZW_OFFSET_INVALID UINT64_MAX
 
 /* Library version (major*100 + minor). */
 int czw_version(void);
 
-/* Bytes of scratch the compressors need for nblocks blocks. */
+/* Bytes of scratch the compressors need for nblocks blocks: a 256-byte
+ * header (its 32-bit word 1, byte 4, holds the device error bits of the last
+ * compress that used it; 0 = every record written), then one fixed-size
+ * slot per block (czw_slot_bytes) and the per-block size words. */
 size_t czw_compress_workspace_bytes(int nblocks, int prec);
 
 /* Upper bound of the record stream size for nblocks blocks. */
@@ -79,10 +88,30 @@ size_t czw_records_bound(int nblocks, int prec);
 
 /* Replaces the loop body of SerializerIO_..._MPI_Simple.h:212-240 (gather a
  * block, WaveletCompressorGeneric::compress at WaveletCompressor.cpp:153-182,
- * frame the record) for every block of an HDF5-ordered field. */
+ * frame the record) for every block of an HDF5-ordered field.  Two stages on
+ * `stream`: the compressor writes each block's payload into its slot of the
+ * workspace, then a framing kernel concatenates the slots into the record
+ * stream and writes the offsets. */
 int czw_compress_field(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype,
                        int zerobits, int shuffle, void *records, uint64_t *offsets, void *workspace,
                        hipStream_t stream);
+
+/* czw_compress_field with a small workspace: scratch of
+ * czw_compress_lowmem_bytes(nblocks, prec) bytes, 256 + 8*nblocks +
+ * min(nblocks, 256) * 114688 bytes, instead of one slot per block (binary32
+ * only; CZW_EUNSUPPORTED for prec 8).  One compressor launch writes the
+ * records itself: each block's record offset, the sum of its predecessors'
+ * record sizes, is found on the device by decoupled look-back, and a block
+ * waits only for blocks taken from the block queue before it.  A block that
+ * waits longer than the look-back limit gives up: its record is not written,
+ * offsets[b] = CZW_OFFSET_INVALID and the scratch's error word gets
+ * CZW_ERR_LOOKBACK (read it with czw_compress_status).  That takes another
+ * kernel holding the device's compute units as long, e.g. one that waits on
+ * this call.  Same records and offsets as czw_compress_field. */
+size_t czw_compress_lowmem_bytes(int nblocks, int prec);
+int czw_compress_field_lowmem(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype,
+                              int zerobits, int shuffle, void *records, uint64_t *offsets, void *scratch,
+                              hipStream_t stream);
 
 /* czw_compress_field for the field's blocks [block0, block0 + nblocks) only
  * (block numbering as in "Field layout"); records/offsets/workspace are sized
@@ -105,6 +134,17 @@ int czw_compress_field_slots(const void *field, int nx, int ny, int nz, int prec
                              int zerobits, void *slots, uint32_t *nbytes, hipStream_t stream);
 int czw_pack_records(const void *slots, size_t slot_bytes, const uint32_t *nbytes, int nblocks, int shuffle,
                      void *records, uint64_t *offsets, hipStream_t stream);
+
+/* Waits for `stream` and returns CZW_EDEVICE if the last compress on this
+ * workspace (or lowmem scratch) reported an error (its error word is
+ * non-zero), else CZW_OK. */
+int czw_compress_status(const void *workspace, hipStream_t stream);
+
+/* Re-reads of a not-ready predecessor's status before a block of
+ * czw_compress_field_lowmem gives up (0 restores the default, 2^20, about a
+ * second).  Returns the previous value.  Applies to calls made after it,
+ * process-wide. */
+uint32_t czw_set_lookback_limit(uint32_t rereads);
 
 /* Same as czw_compress_field for blocks already laid out [nb][z][y][x]. */
 int czw_compress_blocks(const void *blocks, int nblocks, int prec, float eps, int wtype, int zerobits,
@@ -149,6 +189,12 @@ typedef struct czw_field {
 
 #define CZW_BATCH_MAX_FIELDS 64
 
+/* Launch shape of one stage of a batch. */
+typedef struct czw_shape {
+    int groups; /* persistent workgroups; 0 = as many as are resident (never more) */
+    int queue;  /* 1: dynamic block queue; 0: static stride */
+} czw_shape_t;
+
 typedef struct czw_batch {
     const czw_field_t *fields; /* DEVICE copy of the prepared table */
     int nfields;               /* 1 .. CZW_BATCH_MAX_FIELDS */
@@ -159,9 +205,8 @@ typedef struct czw_batch {
     uint64_t *offsets;         /* nblocks + 1 */
     size_t records_bytes;      /* size of the records buffer (decompress bound) */
     uint32_t *err;             /* decompress error bits; cleared by each decompress */
-    int groups;                /* persistent workgroups; 0 = as many as are resident (the binary32
-                                  compress stage uses at most its resident count) */
-    int queue;                 /* 1: dynamic block queue; 0: static stride */
+    czw_shape_t compress;      /* launch shape of the compress stage */
+    czw_shape_t decompress;    /* launch shape of the decompress stage */
 } czw_batch_t;
 
 /* Host: validate a table for Real of prec bytes, fill block0/nblocks, return
@@ -169,7 +214,8 @@ typedef struct czw_batch {
  * per-field entry points have no such limit). */
 int czw_batch_prepare(czw_field_t *fields, int nfields, int prec, int *nblocks);
 
-/* Scratch of a batch: compressor slots and sizes, then 256 B of queue counters. */
+/* Scratch of a batch: the compressor workspace of its nblocks (header word 0
+ * the compress queue, word 1 the error bits, word 2 the decompress queue). */
 size_t czw_batch_workspace_bytes(int nblocks, int prec);
 
 /* Compress every field (fwt, thresholds, survivor streams, records). */
@@ -178,9 +224,11 @@ int czw_batch_compress(const czw_batch_t *batch, hipStream_t stream);
 /* Decompress every field from batch->records into each field's dst. */
 int czw_batch_decompress(const czw_batch_t *batch, hipStream_t stream);
 
-/* Time the launch shapes (groups x queue) of one stage (1 compress,
- * 2 decompress) on this batch and store the fastest in batch->groups/queue;
- * *best_ms receives its time.  Runs the stage several times on `stream`. */
+/* Time the launch shapes (all or half of the stage kernel's resident
+ * workgroups x queue or static order, where the stage has both) of one stage
+ * (1 compress, 2 decompress) on this batch and store the fastest in that
+ * stage's shape only; *best_ms receives its time.  Runs the stage several
+ * times on `stream`. */
 int czw_batch_autotune(czw_batch_t *batch, int stage, hipStream_t stream, float *best_ms);
 
 /* Capture the stages (bit 0 compress, bit 1 decompress) of a batch into an

@@ -20,10 +20,19 @@
 // One block per call keeps the GPU nearly idle; this class exists for code
 // that is written around the per-block object.  Whole fields go through
 // czw_compress_field / czw_decompress_field instead.
+//
+// The reference constructs one object per block on the stack
+// (SerializerIO_WaveletCompression_MPI_Simple.h:214, Reader...h:683), so the
+// object itself holds only host memory: its device buffers belong to the
+// calling thread (one set per device and precision, allocated on the
+// thread's first call and freed when the thread ends) and are shared by every
+// object that thread creates.  Objects of one thread therefore must not be
+// used concurrently on different streams.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
@@ -31,6 +40,61 @@
 #include <string>
 
 #include "czw.h"
+
+namespace czw_adapter {
+
+// Device buffers of one block's compress or decompress.
+struct Scratch {
+    void *block = nullptr, *rec = nullptr, *ws = nullptr;
+    uint64_t *off = nullptr;
+    uint32_t *err = nullptr;
+    bool ready = false;
+    Scratch() = default;
+    Scratch(const Scratch &) = delete;
+    Scratch &operator=(const Scratch &) = delete;
+    ~Scratch()
+    {
+        if (!ready) return;
+        (void)hipFree(block);
+        (void)hipFree(rec);
+        (void)hipFree(ws);
+        (void)hipFree(off);
+        (void)hipFree(err);
+    }
+};
+
+// Buffer sets allocated so far in this process (for tests and accounting).
+inline std::atomic<unsigned long> &allocations()
+{
+    static std::atomic<unsigned long> n{0};
+    return n;
+}
+
+// The calling thread's buffers on the current device for Real of PREC bytes.
+template <int PREC>
+Scratch &scratch()
+{
+    constexpr int MAXDEV = 16;
+    thread_local Scratch sets[MAXDEV];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAXDEV) throw std::runtime_error("czw: no usable device");
+    Scratch &x = sets[dev];
+    if (!x.ready) {
+        auto chk = [](hipError_t e) {
+            if (e != hipSuccess) throw std::runtime_error(std::string("czw: hipMalloc: ") + hipGetErrorString(e));
+        };
+        chk(hipMalloc(&x.block, (size_t)PREC * 32768));
+        chk(hipMalloc(&x.rec, czw_records_bound(1, PREC)));
+        chk(hipMalloc(&x.ws, czw_compress_workspace_bytes(1, PREC)));
+        chk(hipMalloc(reinterpret_cast<void **>(&x.off), 2 * sizeof(uint64_t)));
+        chk(hipMalloc(reinterpret_cast<void **>(&x.err), sizeof(uint32_t)));
+        x.ready = true;
+        allocations() += 1;
+    }
+    return x;
+}
+
+}  // namespace czw_adapter
 
 template <int BS, typename DataType>
 class WaveletCompressorGPU
@@ -44,20 +108,6 @@ public:
     explicit WaveletCompressorGPU(int zerobits = 0, bool shuffle = false, hipStream_t stream = nullptr)
         : zerobits_(zerobits), shuffle_(shuffle), stream_(stream)
     {
-        const int p = (int)sizeof(DataType);
-        check(hipMalloc(&dblock_, sizeof(DataType) * BS3), "hipMalloc");
-        check(hipMalloc(&drec_, czw_records_bound(1, p)), "hipMalloc");
-        check(hipMalloc(&dws_, czw_compress_workspace_bytes(1, p)), "hipMalloc");
-        check(hipMalloc(reinterpret_cast<void **>(&doff_), 2 * sizeof(uint64_t)), "hipMalloc");
-        check(hipMalloc(reinterpret_cast<void **>(&derr_), sizeof(uint32_t)), "hipMalloc");
-    }
-    ~WaveletCompressorGPU()
-    {
-        (void)hipFree(dblock_);
-        (void)hipFree(drec_);
-        (void)hipFree(dws_);
-        (void)hipFree(doff_);
-        (void)hipFree(derr_);
     }
     WaveletCompressorGPU(const WaveletCompressorGPU &) = delete;
     WaveletCompressorGPU &operator=(const WaveletCompressorGPU &) = delete;
@@ -69,14 +119,16 @@ public:
     size_t compress(const float threshold, const bool /*float16*/, int wtype)
     {
         const int p = (int)sizeof(DataType);
-        check(hipMemcpyAsync(dblock_, &data_[0][0][0], sizeof(data_), hipMemcpyHostToDevice, stream_), "copy");
-        status(czw_compress_blocks(dblock_, 1, p, threshold, wtype, zerobits_, shuffle_ ? 1 : 0, drec_, doff_, dws_,
+        czw_adapter::Scratch &d = czw_adapter::scratch<(int)sizeof(DataType)>();
+        check(hipMemcpyAsync(d.block, &data_[0][0][0], sizeof(data_), hipMemcpyHostToDevice, stream_), "copy");
+        status(czw_compress_blocks(d.block, 1, p, threshold, wtype, zerobits_, shuffle_ ? 1 : 0, d.rec, d.off, d.ws,
                                    stream_),
                "czw_compress_blocks");
         uint64_t off[2];
-        check(hipMemcpyAsync(off, doff_, sizeof(off), hipMemcpyDeviceToHost, stream_), "copy");
-        check(hipStreamSynchronize(stream_), "sync");
-        check(hipMemcpyAsync(buf_, drec_, off[1], hipMemcpyDeviceToHost, stream_), "copy");
+        check(hipMemcpyAsync(off, d.off, sizeof(off), hipMemcpyDeviceToHost, stream_), "copy");
+        status(czw_compress_status(d.ws, stream_), "czw_compress_blocks (device)");
+        if (off[1] > sizeof(buf_)) throw std::runtime_error("czw: record larger than a block's bound");
+        check(hipMemcpyAsync(buf_, d.rec, off[1], hipMemcpyDeviceToHost, stream_), "copy");
         check(hipStreamSynchronize(stream_), "sync");
         int32_t n;
         std::memcpy(&n, buf_, 4);
@@ -90,14 +142,16 @@ public:
         const int32_t n = (int32_t)bytes;
         std::memcpy(buf_, &n, 4);
         const uint64_t off[2] = {0, 4 + (uint64_t)bytes};
-        check(hipMemcpyAsync(drec_, buf_, 4 + bytes, hipMemcpyHostToDevice, stream_), "copy");
-        check(hipMemcpyAsync(doff_, off, sizeof(off), hipMemcpyHostToDevice, stream_), "copy");
-        check(hipMemsetAsync(derr_, 0, sizeof(uint32_t), stream_), "memset");
-        status(czw_decompress_blocks(drec_, 4 + bytes, doff_, 1, p, wtype, shuffle_ ? 1 : 0, dblock_, derr_, stream_),
+        const uint32_t zero = 0;
+        czw_adapter::Scratch &d = czw_adapter::scratch<(int)sizeof(DataType)>();
+        check(hipMemcpyAsync(d.rec, buf_, 4 + bytes, hipMemcpyHostToDevice, stream_), "copy");
+        check(hipMemcpyAsync(d.off, off, sizeof(off), hipMemcpyHostToDevice, stream_), "copy");
+        check(hipMemcpyAsync(d.err, &zero, sizeof(zero), hipMemcpyHostToDevice, stream_), "copy");
+        status(czw_decompress_blocks(d.rec, 4 + bytes, d.off, 1, p, wtype, shuffle_ ? 1 : 0, d.block, d.err, stream_),
                "czw_decompress_blocks");
         uint32_t err = 0;
-        check(hipMemcpyAsync(&err, derr_, sizeof(err), hipMemcpyDeviceToHost, stream_), "copy");
-        check(hipMemcpyAsync(&data_[0][0][0], dblock_, sizeof(data_), hipMemcpyDeviceToHost, stream_), "copy");
+        check(hipMemcpyAsync(&err, d.err, sizeof(err), hipMemcpyDeviceToHost, stream_), "copy");
+        check(hipMemcpyAsync(&data_[0][0][0], d.block, sizeof(data_), hipMemcpyDeviceToHost, stream_), "copy");
         check(hipStreamSynchronize(stream_), "sync");
         if (err) throw std::runtime_error("czw: corrupt payload (error bits " + std::to_string(err) + ")");
     }
@@ -126,7 +180,4 @@ private:
     int zerobits_;
     bool shuffle_;
     hipStream_t stream_;
-    void *dblock_ = nullptr, *drec_ = nullptr, *dws_ = nullptr;
-    uint64_t *doff_ = nullptr;
-    uint32_t *derr_ = nullptr;
 };

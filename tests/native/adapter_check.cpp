@@ -3,7 +3,12 @@
 // block in, compress, take the payload; memcpy a payload into
 // compressed_data(), decompress into a block.
 //
-//   adapter_check <f32|f64> <block.raw> <eps> <wtype> <zerobits> <shuffle> <payload.out> <decoded.out>
+//   adapter_check <f32|f64> <block.raw> <eps> <wtype> <zerobits> <shuffle> <payload.out> <decoded.out> [repeat]
+//
+// With repeat, the block then goes through `repeat` more compress/decompress
+// rounds, each with fresh objects (one per block, as the reference's writer
+// constructs them); every round must give the same payload and block, and
+// the line "allocations N" reports the device buffer sets allocated in all.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -39,13 +44,27 @@ static int run(char **argv)
     o = std::fopen(argv[8], "wb");
     if (!o || std::fwrite(out.data(), sizeof(T), out.size(), o) != out.size()) return 3;
     std::fclose(o);
+    const int repeat = argv[9] ? std::atoi(argv[9]) : 0;
+    for (int k = 0; k < repeat; ++k) {
+        WaveletCompressorGPU<32, T> w(zb, sh != 0);
+        w.copy_from(reinterpret_cast<const T(*)[32][32]>(blk.data()));
+        const size_t m = w.compress(eps, false, wtype);
+        if (m != n || std::memcmp(w.compressed_data(), payload.data(), n) != 0) return 5;
+        WaveletCompressorGPU<32, T> r(zb, sh != 0);
+        std::memcpy(r.compressed_data(), payload.data(), n);
+        std::vector<T> again(32768);
+        r.decompress(false, n, wtype, reinterpret_cast<T(*)[32][32]>(again.data()));
+        if (std::memcmp(again.data(), out.data(), sizeof(T) * out.size()) != 0) return 6;
+    }
+    std::printf("allocations %lu\n", czw_adapter::allocations().load());
     return 0;
 }
 
 int main(int argc, char **argv)
 {
-    if (argc != 9) {
-        std::fprintf(stderr, "usage: %s f32|f64 block.raw eps wtype zerobits shuffle payload.out decoded.out\n", argv[0]);
+    if (argc != 9 && argc != 10) {
+        std::fprintf(stderr, "usage: %s f32|f64 block.raw eps wtype zerobits shuffle payload.out decoded.out [repeat]\n",
+                     argv[0]);
         return 1;
     }
     try {

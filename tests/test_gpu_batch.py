@@ -100,9 +100,12 @@ def test_batch_autotune_keeps_results(torch_gpu):
     fs = _fields(torch, seed0=50)
     bt = B.Batch(fs, EPS, wtype=3)
     ms_c = bt.autotune("compress")
+    shape_c = dict(bt.launch_shape["compress"])
     ms_d = bt.autotune("decompress")
     assert ms_c > 0 and ms_d > 0
-    assert bt.launch_shape["groups"] > 0
+    assert shape_c["groups"] > 0 and bt.launch_shape["decompress"]["groups"] > 0
+    # tuning one stage leaves the other stage's shape as it was
+    assert bt.launch_shape["compress"] == shape_c
     bt.compress()
     bt.decompress()
     _check_against_per_field(torch, bt, fs, EPS, 3, 0)
