@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <type_traits>
 
 #include "czw_common.hpp"
@@ -902,6 +903,33 @@ static int compress_rec_f32(const float *src, int nblocks, FieldGeom g, float ep
     return launch_compress_f32<LAYOUT, true>(wt, src, g, nblocks, eps, o, nullptr, 0, 0, s);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free variant of czw_compress_field_lowmem: its scratch is a
+// file-scope device array of fixed size, one copy per GPU, whose address is
+// looked up once per device (hipGetSymbolAddress) and used from then on.
+// Every call on a GPU shares that GPU's array, so calls that overlap on one
+// GPU race; the fill at the start of each call clears what must start at
+// zero (header and status words).
+constexpr int NOALLOC_MAX_BLOCKS = 32768;  // 1024^3 binary32 values
+constexpr size_t NOALLOC_BYTES = WS_HDR + 8 * (size_t)NOALLOC_MAX_BLOCKS + (size_t)MAX_GROUPS * OVF_WORDS * 4;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
+
+static void *scratch_ptr()  // this device's copy of scratch, nullptr if it cannot be found
+{
+    constexpr int MAXDEV = 64;
+    static void *ptr[MAXDEV];
+    static std::mutex mu;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAXDEV) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        ptr[dev] = p;
+    }
+    return ptr[dev];
+}
+
 // Every layout of a single field, both paths.
 static int compress_any(int layout, const void *src, int nblocks, FieldGeom g, int prec, float eps, int wtype,
                         int zerobits, int shuffle, void *records, uint64_t *offsets, void *workspace, bool lowmem,
@@ -1024,6 +1052,30 @@ int czw_compress_field(const void *field, int nx, int ny, int nz, int prec, floa
 }
 
 size_t czw_compress_lowmem_bytes(int nblocks, int prec) { return lowmem_workspace_bytes(nblocks, prec); }
+
+int czw_compress_field_noalloc(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype, int zerobits,
+                               int shuffle, void *records, uint64_t *offsets, hipStream_t stream)
+{
+    if (!field_ok(nx, ny, nz) || !field) return CZW_EINVAL;
+    if (prec != 4) return CZW_EUNSUPPORTED;
+    const int nb = (nx / 32) * (ny / 32) * (nz / 32);
+    if (nb > NOALLOC_MAX_BLOCKS) return CZW_ETOOBIG;
+    void *p = scratch_ptr();
+    if (!p) return CZW_EHIP;
+    const int layout = field_fits_buffer(ny, nz, prec) ? LAYOUT_FIELD : LAYOUT_FIELD_WIDE;
+    return compress_any(layout, field, nb, FieldGeom{nx, ny, nz}, prec, eps, wtype, zerobits, shuffle, records,
+                        offsets, p, true, stream);
+}
+
+int czw_noalloc_errors(uint32_t *bits, hipStream_t stream)
+{
+    void *p = scratch_ptr();
+    if (!p || !bits) return p ? CZW_EINVAL : CZW_EHIP;
+    if (hipMemcpyAsync(bits, static_cast<uint8_t *>(p) + 4, 4, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return CZW_EHIP;
+    return CZW_OK;
+}
 
 int czw_compress_field_lowmem(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype, int zerobits,
                               int shuffle, void *records, uint64_t *offsets, void *scratch, hipStream_t stream)

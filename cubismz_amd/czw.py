@@ -28,7 +28,7 @@ WAVELET_NAMES = {
 }
 
 _STATUS = {0: "ok", -1: "invalid argument", -2: "HIP launch failure", -3: "unsupported combination",
-           -4: "device error (see the error bits)"}
+           -4: "device error (see the error bits)", -5: "more blocks than the fixed scratch holds"}
 ERR_LOOKBACK = 4  # CZW_ERR_LOOKBACK
 OFFSET_INVALID = (1 << 64) - 1  # CZW_OFFSET_INVALID
 
@@ -76,6 +76,10 @@ def load_library():
     L.czw_compress_lowmem_bytes.argtypes = [i, i]
     L.czw_compress_field_lowmem.restype = i
     L.czw_compress_field_lowmem.argtypes = [vp, i, i, i, i, f, i, i, i, vp, vp, vp, vp]
+    L.czw_compress_field_noalloc.restype = i
+    L.czw_compress_field_noalloc.argtypes = [vp, i, i, i, i, f, i, i, i, vp, vp, vp]
+    L.czw_noalloc_errors.restype = i
+    L.czw_noalloc_errors.argtypes = [C.POINTER(C.c_uint32), vp]
     L.czw_set_lookback_limit.restype = C.c_uint32
     L.czw_set_lookback_limit.argtypes = [C.c_uint32]
     _lib = L
@@ -229,6 +233,29 @@ def compress_field_lowmem(field, eps, wtype=3, zerobits=0, shuffle=False, record
     _check(rc, "czw_compress_field_lowmem")
     if check:
         check_workspace(scratch)
+    return records, offsets
+
+
+def compress_field_noalloc(field, eps, wtype, zerobits, shuffle, records, offsets, check=True):
+    """compress_field_lowmem with the library's fixed device scratch: the
+    C call allocates nothing (records and offsets are the caller's).  Calls
+    that overlap on one GPU race (they share the scratch)."""
+    torch = _torch()
+    L = load_library()
+    nx, ny, nz = _field_arg(field, torch)
+    prec = _prec(field.dtype, torch)
+    nblocks = (nx // BLOCK) * (ny // BLOCK) * (nz // BLOCK)
+    _need(records, "records", torch, torch.uint8, records_bound(nblocks, prec), device=field.device)
+    _need(offsets, "offsets", torch, torch.int64, 8 * (nblocks + 1), device=field.device)
+    rc = L.czw_compress_field_noalloc(C.c_void_p(field.data_ptr()), nx, ny, nz, prec, float(eps), int(wtype),
+                                      int(zerobits), int(bool(shuffle)), C.c_void_p(records.data_ptr()),
+                                      C.c_void_p(offsets.data_ptr()), _stream(torch))
+    _check(rc, "czw_compress_field_noalloc")
+    if check:
+        bits = C.c_uint32(0)
+        _check(L.czw_noalloc_errors(C.byref(bits), _stream(torch)), "czw_noalloc_errors")
+        if bits.value:
+            raise CZWError(f"compressor error bits {bits.value:#x}")
     return records, offsets
 
 

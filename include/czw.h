@@ -60,7 +60,8 @@ enum {
     CZW_EINVAL = -1,      /* bad argument (sizes, wtype, zerobits, null) */
     CZW_EHIP = -2,        /* a HIP launch failed */
     CZW_EUNSUPPORTED = -3, /* combination not built */
-    CZW_EDEVICE = -4       /* the device reported an error (czw_compress_status) */
+    CZW_EDEVICE = -4,      /* the device reported an error (czw_compress_status) */
+    CZW_ETOOBIG = -5       /* more blocks than a fixed-size scratch holds */
 };
 
 /* Device error bits (OR-ed): the decompressors write them into *err, the
@@ -112,6 +113,20 @@ size_t czw_compress_lowmem_bytes(int nblocks, int prec);
 int czw_compress_field_lowmem(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype,
                               int zerobits, int shuffle, void *records, uint64_t *offsets, void *scratch,
                               hipStream_t stream);
+
+/* czw_compress_field_lowmem without the scratch argument; no counterpart in
+ * the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the call allocates nothing and uses a fixed device array of the library,
+ * about 28 MiB, one per GPU, held until the process ends.  At most 32768
+ * blocks (a 1024^3 binary32 field); more returns CZW_ETOOBIG; prec 8
+ * returns CZW_EUNSUPPORTED.  Every call on one GPU shares that array, so
+ * calls that overlap on one GPU (different streams, or host threads) race:
+ * serialize them.  czw_noalloc_errors waits for `stream` and returns the
+ * array's error bits (CZW_ERR_LOOKBACK) from the last call on this GPU. */
+int czw_compress_field_noalloc(const void *field, int nx, int ny, int nz, int prec, float eps, int wtype,
+                               int zerobits, int shuffle, void *records, uint64_t *offsets, hipStream_t stream);
+int czw_noalloc_errors(uint32_t *bits, hipStream_t stream);
 
 /* czw_compress_field for the field's blocks [block0, block0 + nblocks) only
  * (block numbering as in "Field layout"); records/offsets/workspace are sized
