@@ -401,7 +401,9 @@ int parse_cz(const std::vector<uint8_t> &file, FileInfo &info, std::string *msg)
         HeaderLUT hl;
         std::memcpy(&hl, file.data() + hstart + (size_t)s * sizeof(HeaderLUT), sizeof(hl));
         const int nc = hl.nchunks;
-        if (nc <= 0 || hl.aggregate_bytes < (uint64_t)nc * 8) return fail("bad HeaderLUT");
+        // the subdomain lies inside the binary ocean before any arithmetic on its size
+        if (nc <= 0 || base > hd || hl.aggregate_bytes > hd - base || hl.aggregate_bytes < (uint64_t)nc * 8)
+            return fail("bad HeaderLUT");
         const uint64_t payload = hl.aggregate_bytes - (uint64_t)nc * 8;
         const uint64_t lut_at = base + payload;
         if (lut_at + (uint64_t)nc * 8 > hd) return fail("LUT outside the binary ocean");
@@ -542,16 +544,26 @@ extern "C" {
 int czh_entries_per_chunk(int prec) { return cz::entries_per_chunk(prec); }
 
 int czh_write(const char *path, int prec, const int bpd[3], int wtype, double threshold, int encoder,
-              const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads)
+              const uint8_t *records, size_t records_bytes, const uint64_t *offsets, int nblocks, int nthreads)
 {
-    return czh_write_subdomains(path, prec, bpd, bpd, wtype, threshold, encoder, records, offsets, nblocks, nthreads);
+    return czh_write_subdomains(path, prec, bpd, bpd, wtype, threshold, encoder, records, records_bytes, offsets,
+                                nblocks, nthreads);
 }
 
 int czh_write_subdomains(const char *path, int prec, const int bpd[3], const int sub_bpd[3], int wtype,
-                         double threshold, int encoder, const uint8_t *records, const uint64_t *offsets, int nblocks,
-                         int nthreads)
+                         double threshold, int encoder, const uint8_t *records, size_t records_bytes,
+                         const uint64_t *offsets, int nblocks, int nthreads)
 {
-    if (!path || !bpd || !sub_bpd || !records || !offsets || (prec != 4 && prec != 8)) return CZH_EINVAL;
+    if (!path || !bpd || !sub_bpd || !records || !offsets || (prec != 4 && prec != 8) || nblocks <= 0)
+        return CZH_EINVAL;
+    // every record inside the buffer: [int32 n][n bytes] between its offset and the next
+    for (int b = 0; b < nblocks; ++b) {
+        if (offsets[b + 1] < offsets[b] || offsets[b + 1] - offsets[b] < 4 || offsets[b + 1] > records_bytes)
+            return CZH_EINVAL;
+        int32_t n;
+        std::memcpy(&n, records + offsets[b], 4);
+        if (n < 0 || 4 + (uint64_t)n > offsets[b + 1] - offsets[b]) return CZH_EINVAL;
+    }
     cz::Params p;
     p.prec = prec;
     for (int d = 0; d < 3; ++d) {

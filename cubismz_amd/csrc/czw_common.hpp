@@ -243,8 +243,9 @@ CZW_DI void iwt16_8(R *A, int t)
             lds_st<16>(A, z * SP_ + x, RP_, r);
         }
         __syncthreads();
-        if (t < 256) {  // level 16 x-pass
-            const int z = t >> 4, y = t & 15;
+        if (t < 256) {  // level 16 x-pass: lanes step through y, or through z
+                        // where rows are 16 banks apart (RP_ = 16 mod 32)
+            const int z = (RP_ % 32 == 16) ? (t & 15) : (t >> 4), y = (RP_ % 32 == 16) ? (t >> 4) : (t & 15);
             lds_ld<16>(A, z * SP_ + y * RP_, 1, r);
             row<16, false, WT, R>(r);
             lds_st<16>(A, z * SP_ + y * RP_, 1, r);

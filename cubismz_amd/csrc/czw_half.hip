@@ -45,11 +45,21 @@ constexpr int NCNT = 8;  // per-block counts: corner, then level-32 codes 1..7
 // x-row q (0..1023) of a block: (z, y) = (q >> 5, q & 31) in the block
 // layout, (y, z) = (q >> 5, q & 31) in the HDF5 field layout (as for
 // binary32); its x-half in H starts here.
-template <int RL>
+template <int RL, int RP_ = RPD, int SP_ = SPD>
 CZW_DI int xrow_half_base(int q)
 {
-    if constexpr (RL == LAYOUT_BLOCKS) return (q >> 5) * SPD + (q & 31) * RPD;
-    else return (q & 31) * SPD + (q >> 5) * RPD;
+    if constexpr (RL == LAYOUT_BLOCKS) return (q >> 5) * SP_ + (q & 31) * RP_;
+    else return (q & 31) * SP_ + (q >> 5) * RP_;
+}
+
+// The decoder's x-row k of thread t: z = t & 31 and y = (t >> 5) + 16k in
+// either layout, so that a half-wave's LDS accesses to its rows' x-halves
+// step through z (one bank apart with SP_ odd).
+template <int RL>
+CZW_DI int dec_xrow(int t, int k)
+{
+    if constexpr (RL == LAYOUT_BLOCKS) return ((t & 31) << 5) | ((t >> 5) + 16 * k);
+    else return t + NT * k;
 }
 
 template <typename R>
@@ -199,8 +209,12 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_compress_h(const
     const int t = threadIdx.x;
     const int G = gridDim.x;
     load_table<LAYOUT>(T, fields, nf);
-    // pass-row owner: (slow, x & 15) = (t >> 4, t & 15), slow = z (y-pass) or y (z-pass)
-    const int xh = t & 15, sl = t >> 4;
+    // pass-row owners: z-pass row (y, x & 15) = (t >> 4, t & 15); y-pass row
+    // (z, x & 15) = (yz, t & 15) with yz = (t >> 5) + 16*((t >> 4) & 1), so
+    // that a half-wave's 32 lanes take z and z + 16 (16 banks apart, as SPD
+    // is 1 modulo 32) instead of z and z + 1 (one bank apart, 2-way conflicts
+    // in binary32)
+    const int xh = t & 15, sl = t >> 4, yz = (t >> 5) + 16 * ((t >> 4) & 1);
     for (int b = blockIdx.x, it = 0; b < nblocks; ++it) {
         if (queue && t == 0) ticket[it & 1] = G + (int)atomicAdd(queue, 1u);
         const BlockRef<R> br = locate<LAYOUT, R>(T, nf, b, src, nullptr, g, eps);
@@ -220,9 +234,9 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_compress_h(const
         __syncthreads();
         {
             R r[32];
-            lds_ld<32>(H, sl * SPD + xh, RPD, r);  // y-pass: row (z, x)
+            lds_ld<32>(H, yz * SPD + xh, RPD, r);  // y-pass: row (z, x)
             row<32, true, WT, R>(r);
-            lds_st<32>(H, sl * SPD + xh, RPD, r);
+            lds_st<32>(H, yz * SPD + xh, RPD, r);
         }
         __syncthreads();
         uint32_t S1;
@@ -268,9 +282,9 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_compress_h(const
         __syncthreads();
         {
             R r[32];
-            lds_ld<32>(H, sl * SPD + xh, RPD, r);
+            lds_ld<32>(H, yz * SPD + xh, RPD, r);
             row<32, true, WT, R>(r);
-            lds_st<32>(H, sl * SPD + xh, RPD, r);
+            lds_st<32>(H, yz * SPD + xh, RPD, r);
         }
         __syncthreads();
         uint32_t S0;
@@ -482,7 +496,13 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
                                                        uint32_t *__restrict__ queue)
 {
     constexpr int RL = row_layout<LAYOUT>();
-    __shared__ R H[HALF_WORDS];
+    // LDS layout of a half: binary32 rows of 16 (no pad) in planes of 513,
+    // so that the y-pass (z and z + 16 in a half-wave) and the z-pass (y and
+    // y + 1) both hit 32 distinct banks; binary64 keeps RPD/SPD (two banks
+    // per element, a quarter-wave per pass)
+    constexpr int RPd = sizeof(R) == 4 ? 16 : RPD;
+    constexpr int SPd = 32 * RPd + 1;
+    __shared__ R H[32 * SPd];
     __shared__ uint32_t M[1024];
     __shared__ uint16_t cnt[2560];  // corner pieces [0,512), level-32 runs 512 + (code-1)*256 + run
     __shared__ uint32_t wsum[NT / 64];
@@ -492,7 +512,8 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
     const int t = threadIdx.x;
     const int G = gridDim.x;
     load_table<LAYOUT>(T, fields, nf);
-    const int xh = t & 15, sl = t >> 4;
+    // z-pass rows (y, x & 15) = (t >> 4, t & 15); y-pass rows (yz, t & 15), see k_compress_h
+    const int xh = t & 15, sl = t >> 4, yz = (t >> 5) + 16 * ((t >> 4) & 1);
     int bn = 0;
     // The record offset of the next block is read one iteration ahead, and a
     // block's size word and mask are read together, so that a block starts
@@ -534,7 +555,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
         // the 4^3 base is always stored (FullWaveletTransform.h:163-176)
         if (t < 16 && (M[(t & 3) + 32 * (t >> 2)] & 0xfu) != 0xfu) flag = 0u;
         uint32_t keep = 0;
-        corner_piece<RPD, SPD>(t, [&](int q, int h, int, int, int word, int shift) {
+        corner_piece<RPd, SPd>(t, [&](int q, int h, int, int, int word, int shift) {
             const uint32_t hm = (1u << h) - 1u;
             keep |= ((t < 8) ? hm : ((M[word] >> shift) & hm)) << (q * h);
         });
@@ -577,7 +598,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
             // before the first LDS store, so that they overlap
             uint32_t idx = cnt[t];
             R v[8];
-            corner_piece<RPD, SPD>(t, [&](int q, int h, int, int, int, int) {
+            corner_piece<RPd, SPd>(t, [&](int q, int h, int, int, int, int) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     if (i < h) {
@@ -587,14 +608,14 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
                     }
                 }
             });
-            corner_piece<RPD, SPD>(t, [&](int q, int h, int addr, int stride, int, int) {
+            corner_piece<RPd, SPd>(t, [&](int q, int h, int addr, int stride, int, int) {
 #pragma unroll
                 for (int i = 0; i < 8; ++i)
                     if (i < h) H[addr + i * stride] = v[q * h + i];
             });
         }
         __syncthreads();
-        iwt16_8<WT, RPD, SPD>(H, t);  // level 8 (x, z, y), level 16 (y, x)
+        iwt16_8<WT, RPd, SPd>(H, t);  // level 8 (x, z, y), level 16 (y, x)
 
         // ---- x < 16: level-16 z-pass on the corner columns, level-32 z- and y-passes
         {
@@ -603,7 +624,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
             R r[32];
             if (y < 16) {
                 R c[16];
-                lds_ld<16>(H, y * RPD + xh, SPD, c);
+                lds_ld<16>(H, y * RPd + xh, SPd, c);
                 row<16, false, WT, R>(c);
 #pragma unroll
                 for (int z = 0; z < 16; ++z) r[z] = c[z];
@@ -625,19 +646,19 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
                 }
             }
             row<32, false, WT, R>(r);
-            lds_st<32>(H, y * RPD + xh, SPD, r);
+            lds_st<32>(H, y * RPd + xh, SPd, r);
         }
         __syncthreads();
         {
             R r[32];
-            lds_ld<32>(H, sl * SPD + xh, RPD, r);  // y-pass: row (z, x)
+            lds_ld<32>(H, yz * SPd + xh, RPd, r);  // y-pass: row (z, x)
             row<32, false, WT, R>(r);
-            lds_st<32>(H, sl * SPD + xh, RPD, r);
+            lds_st<32>(H, yz * SPd + xh, RPd, r);
         }
         __syncthreads();
         R lo[NROW][16];
 #pragma unroll
-        for (int k = 0; k < NROW; ++k) lds_ld<16>(H, xrow_half_base<RL>(t + NT * k), 1, lo[k]);
+        for (int k = 0; k < NROW; ++k) lds_ld<16>(H, xrow_half_base<RL, RPd, SPd>(dec_xrow<RL>(t, k)), 1, lo[k]);
         __syncthreads();
 
         // ---- x >= 16: level-32 z- and y-passes (codes 2, 3 for y < 16; 6, 7 for y >= 16)
@@ -655,25 +676,25 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
                 i += bit;
             }
             row<32, false, WT, R>(r);
-            lds_st<32>(H, y * RPD + xh, SPD, r);
+            lds_st<32>(H, y * RPd + xh, SPd, r);
         }
         __syncthreads();
         {
             R r[32];
-            lds_ld<32>(H, sl * SPD + xh, RPD, r);
+            lds_ld<32>(H, yz * SPd + xh, RPd, r);
             row<32, false, WT, R>(r);
-            lds_st<32>(H, sl * SPD + xh, RPD, r);
+            lds_st<32>(H, yz * SPd + xh, RPd, r);
         }
         __syncthreads();
         // level-32 x-pass on whole rows in registers, then the store
         const BlockRef<R> br = locate<LAYOUT, R>(T, nf, b, nullptr, dst, g, 0.0f);
 #pragma unroll
         for (int k = 0; k < NROW; ++k) {
-            const int q = t + NT * k;
+            const int q = dec_xrow<RL>(t, k);
             R r[32];
 #pragma unroll
             for (int i = 0; i < 16; ++i) r[i] = lo[k][i];
-            lds_ld<16>(H, xrow_half_base<RL>(q), 1, &r[16]);
+            lds_ld<16>(H, xrow_half_base<RL, RPd, SPd>(q), 1, &r[16]);
             row<32, false, WT, R>(r);
             store_xrow_h<R, RL>(br.dst, br.g, br.lb, q, r);
         }

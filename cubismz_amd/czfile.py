@@ -31,8 +31,8 @@ def load_library():
         L = C.CDLL(LIBPATH)
         vp, i = C.c_void_p, C.c_int
         L.czh_entries_per_chunk.argtypes = [i]
-        L.czh_write.argtypes = [C.c_char_p, i, vp, i, C.c_double, i, vp, vp, i, i]
-        L.czh_write_subdomains.argtypes = [C.c_char_p, i, vp, vp, i, C.c_double, i, vp, vp, i, i]
+        L.czh_write.argtypes = [C.c_char_p, i, vp, i, C.c_double, i, vp, C.c_size_t, vp, i, i]
+        L.czh_write_subdomains.argtypes = [C.c_char_p, i, vp, vp, i, C.c_double, i, vp, C.c_size_t, vp, i, i]
         L.czh_open.restype = vp
         L.czh_open.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t]
         L.czh_close.argtypes = [vp]
@@ -65,9 +65,11 @@ def write(path, records, offsets, bpd, prec, wtype, threshold, encoder="zlib", n
     nblocks = int(b.prod())
     if off.size != nblocks + 1:
         raise CzFileError("offsets must have nblocks+1 entries")
+    if np.any(np.diff(off.astype(np.int64)) < 4) or int(off[-1]) > rec.size:
+        raise CzFileError("offsets must increase by at least 4 and stay within the records")
     rc = load_library().czh_write_subdomains(os.fsencode(path), int(prec), b.ctypes.data, s.ctypes.data, int(wtype),
-                                             float(threshold), ENCODERS[encoder], rec.ctypes.data, off.ctypes.data,
-                                             nblocks, int(nthreads))
+                                             float(threshold), ENCODERS[encoder], rec.ctypes.data, rec.size,
+                                             off.ctypes.data, nblocks, int(nthreads))
     if rc:
         raise CzFileError(f"czh_write_subdomains failed ({rc})")
 

@@ -35,9 +35,12 @@ int czh_entries_per_chunk(int prec);
 /* Write a single-subdomain .cz file from a host record stream in block order
  * (block b = ix + bpd[0]*(iy + bpd[1]*iz)).  wtype < 0 records "Wavelets: none"
  * (raw blocks).  threshold is printed as the Real of the given precision.
- * nthreads <= 0 keeps the OpenMP default for the per-chunk deflate. */
+ * nthreads <= 0 keeps the OpenMP default for the per-chunk deflate.
+ * records holds records_bytes bytes; CZH_EINVAL unless every record lies in
+ * it (offsets non-decreasing, offsets[nblocks] <= records_bytes) and its size
+ * word fits between its offset and the next. */
 int czh_write(const char *path, int prec, const int bpd[3], int wtype, double threshold, int encoder,
-              const uint8_t *records, const uint64_t *offsets, int nblocks, int nthreads);
+              const uint8_t *records, size_t records_bytes, const uint64_t *offsets, int nblocks, int nthreads);
 
 /* The file that nranks = prod(bpd[d] / sub_bpd[d]) writer ranks, each holding
  * a subdomain of sub_bpd blocks, write together (the MPI writer of
@@ -47,8 +50,8 @@ int czh_write(const char *path, int prec, const int bpd[3], int wtype, double th
  * in MPI's Cartesian order (last coordinate fastest) and each one chunks its
  * own blocks, x fastest.  sub_bpd == bpd is czh_write. */
 int czh_write_subdomains(const char *path, int prec, const int bpd[3], const int sub_bpd[3], int wtype,
-                         double threshold, int encoder, const uint8_t *records, const uint64_t *offsets, int nblocks,
-                         int nthreads);
+                         double threshold, int encoder, const uint8_t *records, size_t records_bytes,
+                         const uint64_t *offsets, int nblocks, int nthreads);
 
 /* Read and parse a .cz file (any number of subdomains).  Returns NULL on
  * error with a message in errbuf. */

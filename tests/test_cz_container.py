@@ -8,6 +8,8 @@ the uncompressed data.  Here the per-block payloads come from the oracle; the
 GPU versions of these tests live in test_gpu_tools.py.
 """
 import ctypes as C
+import hashlib
+import json
 import lzma
 import os
 import struct
@@ -47,6 +49,49 @@ def test_published_known_answer(oracle, host, tmp_path, prec):
     dec = oracle.decompress_field(slots, nb, d.shape, 3, prec)
     head, vals = oracle.res_line(dec, d, size)
     assert vals == PUBLISHED[prec], (vals, size)
+
+
+REF_CZ = json.load(open(os.path.join(GOLD, "ref_cz.json")))
+
+
+def _arg(args, flag, default=None):
+    return args[args.index(flag) + 1] if flag in args else default
+
+
+def build_case_file(oracle, case, path):
+    """The .cz of a reference case (tests/golden/ref_cz.json) from this
+    repository's writer, with the oracle's payloads (or raw blocks)."""
+    args = case["args"]
+    prec = "f64" if _arg(args, "-precision") == "double" else "f32"
+    s = 8 if prec == "f64" else 4
+    d = demo(prec)
+    sub = (2, 2, 2) if case["ranks"] == 8 else (4, 4, 4)
+    if "-nowavelets" in args:
+        # raw blocks [z][y][x] in block order (x fastest), "Wavelets: none"
+        blocks = np.stack([np.ascontiguousarray(d[32 * ix:32 * ix + 32, 32 * iy:32 * iy + 32,
+                                                  32 * iz:32 * iz + 32].transpose(2, 1, 0))
+                           for iz in range(4) for iy in range(4) for ix in range(4)])
+        rec, off = czfile.records_from_slots(blocks.reshape(64, -1).view(np.uint8), np.full(64, 32768 * s))
+        czfile.write(str(path), rec, off, (4, 4, 4), s, -1, 0.0, encoder="none", sub_bpd=sub)
+        return
+    eps, wt = float(_arg(args, "-threshold")), int(_arg(args, "-wtype"))
+    zb, sh = int(_arg(args, "-zerobits", 0)), "-shuffle" in args
+    slots, nb = oracle.compress_field(d, eps, wt, zb, sh)
+    rec, off = czfile.records_from_slots(slots, nb)
+    czfile.write(str(path), rec, off, (4, 4, 4), s, wt, eps, encoder=_arg(args, "-encoder", "zlib"), sub_bpd=sub)
+
+
+@pytest.mark.parametrize("name", sorted(REF_CZ))
+def test_file_bytes_equal_reference_tools(oracle, host, tmp_path, name):
+    """Byte for byte the .cz that the reference's own hdf2cz writes (built
+    from its sources, oracle/Makefile "tools"; tests/golden/make_ref_cz.py):
+    one rank and the 2x2x2 grid of genref.sh:27, wavelet types 1-3, zerobits,
+    shuffle, LZ4, both precisions and the uncompressed files."""
+    case = REF_CZ[name]
+    path = tmp_path / f"{name}.cz"
+    build_case_file(oracle, case, path)
+    data = path.read_bytes()
+    assert (len(data), hashlib.sha256(data).hexdigest()) == (case["bytes"], case["sha256"])
 
 
 def test_layout_and_header_text(oracle, host, tmp_path):
@@ -146,6 +191,14 @@ def test_reader_rejects_corruption(oracle, host, tmp_path):
         p.write_bytes(data)
         with pytest.raises(czfile.CzFileError):
             czfile.CzFile(str(p))
+    # a HeaderLUT whose aggregate size runs past the file (a uint64 that
+    # would wrap base + size): refused before any chunk is located
+    lut = good.index(b"START-BINARY-LUT==============\n") + len(b"START-BINARY-LUT==============\n")
+    for agg in ((1 << 64) - 8, hd + 1):
+        p = tmp_path / f"bad_lut_{agg}.cz"
+        p.write_bytes(good[:lut] + struct.pack("<Q", agg) + good[lut + 8:])
+        with pytest.raises(czfile.CzFileError):
+            czfile.CzFile(str(p))
     # a corrupted chunk must fail to inflate (the reference accepts it:
     # CompressionEncoders.h:64 treats negative inflate codes as success)
     bad = bytearray(good)
@@ -155,6 +208,28 @@ def test_reader_rejects_corruption(oracle, host, tmp_path):
     p.write_bytes(bytes(bad))
     with czfile.CzFile(str(p)) as f, pytest.raises(czfile.CzFileError):
         f.records()
+
+
+def test_writer_rejects_bad_offsets(oracle, host, tmp_path):
+    """Offsets that decrease, leave the records buffer or cut a record short
+    are refused before the encoder reads anything."""
+    d = demo("f32")[:64, :64, :32]
+    slots, nb = oracle.compress_field(np.ascontiguousarray(d), 1e-3, 1)
+    rec, off = czfile.records_from_slots(slots, nb)
+    bad = {"decreasing": np.concatenate([off[:2], off[1:2], off[3:]]),
+           "past_end": np.concatenate([off[:-1], [off[-1] + 64]]),
+           "short": off.copy()}
+    bad["short"][1] -= 8
+    for name, o in bad.items():
+        with pytest.raises(czfile.CzFileError):
+            czfile.write(str(tmp_path / f"{name}.cz"), rec, o, (2, 2, 1), 4, 1, 1e-3)
+    # past the end of a shorter buffer, checked by the C entry point itself
+    lib = czfile.load_library()
+    b = np.array([2, 2, 1], np.int32)
+    recs = np.ascontiguousarray(rec)
+    o = np.ascontiguousarray(off, np.uint64)
+    assert lib.czh_write_subdomains(str(tmp_path / "c.cz").encode(), 4, b.ctypes.data, b.ctypes.data, 1, 1e-3, 0,
+                                    recs.ctypes.data, recs.size - 1, o.ctypes.data, 4, 0) != 0
 
 
 def test_multi_subdomain_reader(oracle, host, tmp_path):

@@ -153,3 +153,73 @@ def test_random_batches(torch_gpu, case):
         assert torch.equal(boff, off) and torch.equal(brec, rec[: int(off[-1])]), k
         dec = czw.decompress_field(rec, off, f.shape, dtype, wtype, shuffle)
         assert torch.equal(bt.outputs[k].view(iv), dec.view(iv)), k
+
+
+def test_batch_64_fields_of_the_benchmark_shapes(torch_gpu, oracle):
+    """A full table: 64 fields with edges from {64, ..., 384} and per-field
+    eps log-uniform in [1e-4, 1e-2] (the batched benchmark's generator), every
+    field equal to its per-field calls, one small field against the oracle."""
+    torch = torch_gpu
+    shapes, eps = fields.batched_shapes(64, seed=7)
+    print(f"[case] 64-field batch, {sum(int(np.prod(s)) for s in shapes) // 32768} blocks", flush=True)
+    fs = [fields.bubble_field(s, seed=100 + k, nbubbles=fields.bubbles_for(s), dtype=torch.float32, device="cuda")
+          for k, s in enumerate(shapes)]
+    bt = B.Batch(fs, [float(e) for e in eps], wtype=1)
+    bt.compress()
+    bt.decompress()
+    _check_against_per_field(torch, bt, fs, [float(e) for e in eps], 1, False)
+    k = int(np.argmin([np.prod(s) for s in shapes]))
+    fh = fs[k].cpu().numpy()
+    slots, nb = oracle.compress_field(fh, float(eps[k]), 1)
+    rec, off = (t.cpu().numpy() for t in bt.field_records(k))
+    for b in range(len(nb)):
+        o = int(off[b])
+        assert np.array_equal(rec[o + 4:o + 4 + int(nb[b])], slots[b, :nb[b]]), b
+    ref = oracle.decompress_field(slots, nb, fh.shape, 1, "f32")
+    assert np.array_equal(bt.outputs[k].cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("dtype_name", ["float32", "float64"])
+@pytest.mark.parametrize("queue", [0, 1])
+def test_batch_more_groups_than_resident(torch_gpu, dtype_name, queue):
+    """A launch shape far above the resident workgroups (4096) is clamped to
+    them, for both stages, in static and queue order."""
+    torch = torch_gpu
+    dtype = getattr(torch, dtype_name)
+    shapes = SHAPES[:5]
+    fs = [fields.bubble_field(s, seed=300 + k, nbubbles=5, dtype=dtype, device="cuda") for k, s in enumerate(shapes)]
+    bt = B.Batch(fs, EPS[:5], wtype=3, shuffle=True)
+    bt.set_launch(4096, queue)
+    print(f"[case] batch {dtype_name} groups=4096 queue={queue}", flush=True)
+    bt.compress()
+    bt.decompress()
+    torch.cuda.synchronize()
+    iv = torch.int32 if dtype == torch.float32 else torch.int64
+    for k, f in enumerate(fs):
+        rec, off = czw.compress_field(f, EPS[k], 3, 0, True)
+        brec, boff = bt.field_records(k)
+        assert torch.equal(boff, off) and torch.equal(brec, rec[: int(off[-1])]), k
+        dec = czw.decompress_field(rec, off, f.shape, dtype, 3, True)
+        assert torch.equal(bt.outputs[k].view(iv), dec.view(iv)), k
+
+
+def test_batch_f64_autotuned_both_stages(torch_gpu):
+    torch = torch_gpu
+    shapes = [(64, 32, 96), (32, 32, 32), (96, 64, 32)]
+    eps = [1e-5, 1e-7, 1e-4]
+    fs = [fields.bubble_field(s, seed=60 + k, nbubbles=4, dtype=torch.float64, device="cuda")
+          for k, s in enumerate(shapes)]
+    bt = B.Batch(fs, eps, wtype=2)
+    bt.autotune("compress")
+    shape_c = dict(bt.launch_shape["compress"])
+    bt.autotune("decompress")
+    assert bt.launch_shape["compress"] == shape_c
+    bt.compress()
+    bt.decompress()
+    torch.cuda.synchronize()
+    for k, f in enumerate(fs):
+        rec, off = czw.compress_field(f, eps[k], 2, 0, False)
+        brec, boff = bt.field_records(k)
+        assert torch.equal(boff, off) and torch.equal(brec, rec[: int(off[-1])]), k
+        dec = czw.decompress_field(rec, off, f.shape, torch.float64, 2, False)
+        assert torch.equal(bt.outputs[k].view(torch.int64), dec.view(torch.int64)), k

@@ -32,7 +32,7 @@ def _check_records(rec, off, slots, nbytes):
 
 @pytest.mark.parametrize("wtype", [1, 2, 3])
 @pytest.mark.parametrize("zerobits,shuffle", [(0, False), (4, False), (0, True), (16, True)])
-@pytest.mark.parametrize("eps", [1e-3, 1e-6])
+@pytest.mark.parametrize("eps", [1e-3, 1e-6, 1e-8])  # 1e-8: the 1024^3 scale-up's densest setting
 def test_field_compress_and_decode_bitexact_f64(torch_gpu, oracle, wtype, zerobits, shuffle, eps):
     torch = torch_gpu
     print(f"[case] f64 field {SHAPE} wtype={wtype} zb={zerobits} sh={shuffle} eps={eps}", flush=True)

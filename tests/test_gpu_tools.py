@@ -1,6 +1,8 @@
 """End-to-end CLI tools on the GPU: hdf2cz -> .cz -> cz2diff, against the
 reference's published known answers (Tests/Test1/cselab_ref_run_all.txt:12:
 demo.h5, wavelet type 3, threshold 5e-5, zlib), and cz2hdf round trips."""
+import hashlib
+import json
 import lzma
 import os
 import subprocess
@@ -17,6 +19,9 @@ BIN = os.path.join(ROOT, "cubismz_amd", "bin")
 GOLD = os.path.join(ROOT, "tests", "golden")
 PUBLISHED_F32 = ("RES:       256.23 8.722263e-04 8.271610e-05 3.972721e-05 1.046102e-04 3.611734e-08"
                  "       0.1249      78.2226")
+# Tests/Test1_dp/cselab_ref_run_all.txt:12 (demo_dp.h5, the double build)
+PUBLISHED_F64 = ("RES:       348.36 8.717107e-04 8.271310e-05 3.972577e-05 1.046065e-04 3.611606e-08"
+                 "       0.1837      78.2229")
 
 
 def _tool(name, *args):
@@ -45,6 +50,57 @@ def test_hdf2cz_cz2diff_published_known_answer(demo_raw, tmp_path):
     out = _tool("cz2diff", "-czfile1", str(cz), "-wtype", "3", "-rawfile2", str(demo_raw))
     res = [l for l in out.splitlines() if l.startswith("RES:")]
     assert res[-1] == PUBLISHED_F32, out
+
+
+@pytest.fixture(scope="module")
+def demo_h5(tmp_path_factory):
+    """The reference's own inputs, demo.h5 and demo_dp.h5 from
+    Tests/Data/data.tar.xz (committed xz-compressed as data)."""
+    exe = os.path.join(BIN, "hdf2cz")
+    if os.path.exists(exe) and "hdf5" not in subprocess.run(["ldd", exe], capture_output=True, text=True).stdout:
+        pytest.skip("tools built without HDF5")
+    d = tmp_path_factory.mktemp("h5")
+    out = {}
+    for prec, name in (("float", "demo.h5"), ("double", "demo_dp.h5")):
+        p = d / name
+        p.write_bytes(lzma.decompress(open(os.path.join(GOLD, name + ".xz"), "rb").read()))
+        out[prec] = p
+    return out
+
+
+@pytest.mark.parametrize("prec", ["float", "double"])
+def test_reference_h5_published_known_answer(demo_h5, tmp_path, prec):
+    """The reference's Test1 / Test1_dp workflow on its own HDF5 inputs:
+    genref.sh's uncompressed ref.cz, test_wavz.sh's hdf2cz (wavelet type 3,
+    threshold 5e-5) and cz2diff; the RES line is the published one
+    (Tests/Test1/cselab_ref_run_all.txt:12, Tests/Test1_dp/...:12)."""
+    h5 = str(demo_h5[prec])
+    extra = ["-precision", "double"] if prec == "double" else []
+    grid = ["-bpdx", "4", "-bpdy", "4", "-bpdz", "4"]
+    _tool("hdf2cz", *grid, "-h5file", h5, "-czfile", "ref.cz", "-fpath", str(tmp_path), "-nowavelets", "-encoder",
+          "none", *extra)
+    _tool("hdf2cz", *grid, "-h5file", h5, "-czfile", "tmp.cz", "-fpath", str(tmp_path), "-threshold", "0.00005",
+          "-wtype", "3", *extra)
+    out = _tool("cz2diff", "-czfile1", str(tmp_path / "tmp.cz"), "-wtype", "3", "-czfile2", str(tmp_path / "ref.cz"))
+    want = PUBLISHED_F64 if prec == "double" else PUBLISHED_F32
+    assert [l for l in out.splitlines() if l.startswith("RES:")][-1] == want, out
+
+
+REF_CZ = json.load(open(os.path.join(GOLD, "ref_cz.json")))
+
+
+@pytest.mark.parametrize("name", sorted(REF_CZ))
+def test_hdf2cz_writes_the_reference_tools_bytes(demo_h5, tmp_path, name):
+    """This repository's hdf2cz on the reference's own HDF5 inputs writes,
+    byte for byte, the .cz that the reference's own hdf2cz writes
+    (tests/golden/ref_cz.json: one rank and the 2x2x2 process grid of
+    genref.sh:27, wavelet types 1-3, zerobits, shuffle, LZ4, both precisions,
+    uncompressed files)."""
+    case = REF_CZ[name]
+    h5 = demo_h5["double" if case["input"] == "demo_dp.h5" else "float"]
+    _tool("hdf2cz", "-h5file", str(h5), "-czfile", "out.cz", "-fpath", str(tmp_path), *case["args"])
+    data = (tmp_path / "out.cz").read_bytes()
+    assert (len(data), hashlib.sha256(data).hexdigest()) == (case["bytes"], case["sha256"])
 
 
 def test_raw_reference_cz_and_cz2hdf_roundtrip(demo_raw, tmp_path):
