@@ -325,16 +325,23 @@ static bool parse_kv(const std::string &line, const char *key, std::string &val)
     return true;
 }
 
-int parse_cz(const std::vector<uint8_t> &file, FileInfo &info, std::string *msg)
+static uint32_t bswap_u32(uint32_t v) { return __builtin_bswap32(v); }
+static uint64_t bswap_u64(uint64_t v) { return __builtin_bswap64(v); }
+
+int parse_cz(const std::vector<uint8_t> &file, FileInfo &info, std::string *msg, bool swap)
 {
     auto fail = [&](const char *m) {
         if (msg) *msg = m;
         return -1;
     };
+    auto fix32 = [&](int32_t v) { return swap ? (int32_t)bswap_u32((uint32_t)v) : v; };
+    auto fix64 = [&](uint64_t v) { return swap ? bswap_u64(v) : v; };
+    info.swap = swap;
     const size_t ocean = sizeof(uint64_t) + std::strlen(OCEAN_TITLE);
     if (file.size() < ocean) return fail("file too short");
     uint64_t hd;
     std::memcpy(&hd, file.data(), sizeof(hd));
+    hd = fix64(hd);
     if (hd < ocean || hd >= file.size()) return fail("bad header displacement");
     if (std::memcmp(file.data() + 8, OCEAN_TITLE, std::strlen(OCEAN_TITLE)) != 0) return fail("missing ocean title");
     info.header_displacement = hd;
@@ -350,7 +357,7 @@ int parse_cz(const std::vector<uint8_t> &file, FileInfo &info, std::string *msg)
     bool have_blocks = false, have_sub = false;
     while (std::getline(hs, line)) {
         if (parse_kv(line, "Endianess:", v)) {
-            if (v != "little") return fail("big-endian files are not supported");
+            if (v != "little" && !swap) return fail("big-endian file: read it with -swap");
         } else if (parse_kv(line, "sizeofReal:", v)) p.prec = std::atoi(v.c_str());
         else if (parse_kv(line, "sizeofsize_t:", v)) sizeof_st = std::atoi(v.c_str());
         else if (parse_kv(line, "sizeofBlockMetadata:", v)) sizeof_bm = std::atoi(v.c_str());
@@ -393,6 +400,14 @@ int parse_cz(const std::vector<uint8_t> &file, FileInfo &info, std::string *msg)
 
     std::vector<BlockMetadata> mb(nb);
     std::memcpy(mb.data(), file.data() + mstart, (size_t)nb * sizeof(BlockMetadata));
+    if (swap)
+        for (auto &m : mb) {
+            m.idcompression = fix32(m.idcompression);
+            m.subid = fix32(m.subid);
+            m.ix = fix32(m.ix);
+            m.iy = fix32(m.iy);
+            m.iz = fix32(m.iz);
+        }
     // per-subdomain look-up tables (Reader_WaveletCompression.h:439-522)
     uint64_t base = ocean;
     info.chunk_start.clear();
@@ -400,6 +415,8 @@ int parse_cz(const std::vector<uint8_t> &file, FileInfo &info, std::string *msg)
     for (int s = 0, cur = 0; s < nsub; ++s) {
         HeaderLUT hl;
         std::memcpy(&hl, file.data() + hstart + (size_t)s * sizeof(HeaderLUT), sizeof(hl));
+        hl.aggregate_bytes = fix64(hl.aggregate_bytes);
+        hl.nchunks = fix32(hl.nchunks);
         const int nc = hl.nchunks;
         // the subdomain lies inside the binary ocean before any arithmetic on its size
         if (nc <= 0 || base > hd || hl.aggregate_bytes > hd - base || hl.aggregate_bytes < (uint64_t)nc * 8)
@@ -409,6 +426,7 @@ int parse_cz(const std::vector<uint8_t> &file, FileInfo &info, std::string *msg)
         if (lut_at + (uint64_t)nc * 8 > hd) return fail("LUT outside the binary ocean");
         std::vector<uint64_t> lut(nc);
         std::memcpy(lut.data(), file.data() + lut_at, (size_t)nc * 8);
+        for (auto &l : lut) l = fix64(l);
         const size_t g0 = info.chunk_start.size();
         for (int i = 0; i < nc; ++i) {
             const uint64_t end = (i + 1 < nc) ? lut[i + 1] : payload;
@@ -474,6 +492,19 @@ int gather_records(const std::vector<uint8_t> &file, const FileInfo &info, int n
         while (o + 4 <= raw[c].size()) {
             int32_t n;
             std::memcpy(&n, raw[c].data() + o, 4);
+            if (info.swap) {  // the size word, then the record's 4-byte units from byte 4096 on
+                n = (int32_t)bswap_u32((uint32_t)n);
+                if (n >= 0 && o + 4 + (uint64_t)n <= raw[c].size()) {
+                    uint8_t *r = records.data() + cbase[c] + o;
+                    std::memcpy(r, &n, 4);
+                    for (uint64_t i = 4096; i + 4 <= (uint64_t)n; i += 4) {
+                        uint32_t w;
+                        std::memcpy(&w, r + 4 + i, 4);
+                        w = bswap_u32(w);
+                        std::memcpy(r + 4 + i, &w, 4);
+                    }
+                }
+            }
             if (n < 0 || o + 4 + (uint64_t)n > raw[c].size()) {
                 if (msg) *msg = "record size word runs past its chunk";
                 return -3;
@@ -579,12 +610,14 @@ int czh_write_subdomains(const char *path, int prec, const int bpd[3], const int
     return cz::write_file(path, file) ? CZH_OK : CZH_EIO;
 }
 
-czh_file *czh_open(const char *path, char *errbuf, size_t errlen)
+czh_file *czh_open(const char *path, char *errbuf, size_t errlen) { return czh_open_ex(path, 0, errbuf, errlen); }
+
+czh_file *czh_open_ex(const char *path, int flags, char *errbuf, size_t errlen)
 {
     czh_file *h = new czh_file;
     std::string msg;
     if (!path || !cz::read_file(path, h->bytes)) msg = "cannot read file";
-    else if (cz::parse_cz(h->bytes, h->info, &msg) != 0) {
+    else if (cz::parse_cz(h->bytes, h->info, &msg, (flags & CZH_OPEN_SWAP) != 0) != 0) {
     } else return h;
     if (errbuf && errlen) std::snprintf(errbuf, errlen, "%s", msg.c_str());
     delete h;

@@ -155,11 +155,17 @@ struct FileInfo {
     std::vector<CompressedBlock> idx2chunk;  // per block id ix + bx*(iy + by*iz)
     std::vector<uint64_t> chunk_start;       // absolute file offsets
     std::vector<uint64_t> chunk_size;
+    bool swap = false;                       // the file's binary fields are byte-swapped
 };
 
 // Parse header, metadata and look-up tables (Reader_WaveletCompression.h:255-560).
-// Returns 0 or a negative error; *msg receives a description.
-int parse_cz(const std::vector<uint8_t> &file, FileInfo &info, std::string *msg);
+// Returns 0 or a negative error; *msg receives a description.  swap: the
+// file was written with the other byte order (the tools' -swap, Reader_
+// WaveletCompression.h:133-225,280,421,449,478): the header displacement,
+// metadata, HeaderLUTs and chunk offsets are byte-reversed on reading, and
+// gather_records reverses each record's size word and, in 4-byte units, its
+// bytes from 4096 on (:605-620; raw blocks included, as the reference does).
+int parse_cz(const std::vector<uint8_t> &file, FileInfo &info, std::string *msg, bool swap = false);
 
 // Inflate every chunk once and gather all block records in block order:
 // records receives the concatenated inflated chunks and offsets[b] the

@@ -171,7 +171,8 @@ bool write_h5(const std::string &path, const Field &f, int nch = 1)
 // all wavelet blocks decoded by one device launch.  The file does not record
 // the byte shuffle (the reference selects it at compile time, _USE_SHUFFLE3_,
 // for its writer and reader alike), so the caller says whether it was used.
-bool decode_cz(const std::string &path, int expect_wtype, Field &f, bool verbose, bool shuffle)
+// swap reads a file of the other byte order (the tools' -swap).
+bool decode_cz(const std::string &path, int expect_wtype, Field &f, bool verbose, bool shuffle, bool swap)
 {
     std::vector<uint8_t> bytes;
     cz::FileInfo info;
@@ -180,7 +181,7 @@ bool decode_cz(const std::string &path, int expect_wtype, Field &f, bool verbose
         std::fprintf(stderr, "cannot read %s\n", path.c_str());
         return false;
     }
-    if (cz::parse_cz(bytes, info, &msg)) {
+    if (cz::parse_cz(bytes, info, &msg, swap)) {
         std::fprintf(stderr, "%s: %s\n", path.c_str(), msg.c_str());
         return false;
     }
@@ -400,21 +401,18 @@ int tool_main(int argc, char **argv)
     const std::string in[3] = {a.str("-czfile", "none"), a.str("-czfile2", "none"), a.str("-czfile3", "none")};
     if (a.has("-help") || in[0] == "none" || (!a.has("-h5file") && !a.has("-rawfile"))) {
         std::printf("Usage: %s -czfile <cz file> -h5file <h5 basefilename> | -rawfile <raw file> [-wtype <wt>] "
-                    "[-czfile2 <cz file>] [-czfile3 <cz file>] [-xs/-xe/-ys/-ye/-zs/-ze <block>] [-shuffle]\n",
+                    "[-czfile2 <cz file>] [-czfile3 <cz file>] [-xs/-xe/-ys/-ye/-zs/-ze <block>] [-shuffle] [-swap]\n",
                     argv[0]);
         return 1;
     }
-    if (a.has("-swap")) {
-        std::fprintf(stderr, "cz2hdf: -swap (big-endian files) is not supported\n");
-        return 1;
-    }
+    const bool swap = a.has("-swap");  // cz2hdf.cpp:79
     const double t0 = now();
     const bool shuffle = a.has("-shuffle") && a.str("-shuffle", "1") != "0";
     Field ch[3];
     int nch = 0;
     for (int c = 0; c < 3; ++c) {
         if (in[c] == "none") continue;
-        if (!decode_cz(in[c], a.i("-wtype", -1), ch[nch], true, shuffle)) return 1;
+        if (!decode_cz(in[c], a.i("-wtype", -1), ch[nch], true, shuffle, swap)) return 1;
         if (nch && (std::memcmp(ch[nch].n, ch[0].n, sizeof(ch[0].n)) || ch[nch].prec != ch[0].prec)) {
             std::fprintf(stderr, "cz2hdf: %s differs in extent or precision from %s\n", in[c].c_str(), in[0].c_str());
             return 1;
@@ -498,19 +496,20 @@ int tool_main(int argc, char **argv)
     Args a(argc, argv);
     const std::string f1n = a.str("-czfile1", "none"), f2n = a.str("-czfile2", "none");
     if (a.has("-help") || f1n == "none" || (f2n == "none" && !a.has("-rawfile2"))) {
-        std::printf("Usage: %s -czfile1 <cz file1> [-wtype <wt>] -czfile2 <cz reference file2> [-shuffle]\n",
+        std::printf("Usage: %s -czfile1 <cz file1> [-wtype <wt>] -czfile2 <cz reference file2> [-shuffle] [-swap]\n",
                     argv[0]);
         return 1;
     }
     const double init_t0 = now();
     Field A, B;
     const bool shuffle = a.has("-shuffle") && a.str("-shuffle", "1") != "0";  // both files, as in one build
-    if (!decode_cz(f1n, a.i("-wtype", 3), A, false, shuffle)) return 1;
+    const bool swap = a.has("-swap");  // both readers, cz2diff.cpp:70-74
+    if (!decode_cz(f1n, a.i("-wtype", 3), A, false, shuffle, swap)) return 1;
     if (a.has("-rawfile2")) {
         B.prec = A.prec;
         std::memcpy(B.n, A.n, sizeof(B.n));
         if (!read_raw(a.str("-rawfile2", ""), B)) return 1;
-    } else if (!decode_cz(f2n, -1, B, false, shuffle)) {
+    } else if (!decode_cz(f2n, -1, B, false, shuffle, swap)) {
         return 1;
     }
     const double init_t1 = now();

@@ -35,6 +35,8 @@ def load_library():
         L.czh_write_subdomains.argtypes = [C.c_char_p, i, vp, vp, i, C.c_double, i, vp, C.c_size_t, vp, i, i]
         L.czh_open.restype = vp
         L.czh_open.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t]
+        L.czh_open_ex.restype = vp
+        L.czh_open_ex.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_size_t]
         L.czh_close.argtypes = [vp]
         L.czh_info.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.czh_file_bytes.restype = C.c_uint64
@@ -132,13 +134,19 @@ def deflate(data):
     return out[:n.value].tobytes()
 
 
-class CzFile:
-    """A parsed .cz file."""
+OPEN_SWAP = 1
 
-    def __init__(self, path):
+
+class CzFile:
+    """A parsed .cz file.  swap=True reads a file of the other byte order, as
+    the reference readers' swapbytes flag (Reader_WaveletCompression.h:133-225,
+    280, 478, 596-620): header displacement, metadata, LUTs and each record's
+    size word and 4-byte units from payload byte 4096 on are byte-reversed."""
+
+    def __init__(self, path, swap=False):
         L = load_library()
         err = C.create_string_buffer(256)
-        h = L.czh_open(os.fsencode(path), err, 256)
+        h = L.czh_open_ex(os.fsencode(path), OPEN_SWAP if swap else 0, err, 256)
         if not h:
             raise CzFileError(f"{path}: {err.value.decode()}")
         self._h = h

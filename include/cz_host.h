@@ -56,6 +56,13 @@ int czh_write_subdomains(const char *path, int prec, const int bpd[3], const int
 /* Read and parse a .cz file (any number of subdomains).  Returns NULL on
  * error with a message in errbuf. */
 czh_file *czh_open(const char *path, char *errbuf, size_t errlen);
+
+/* czh_open with flags: CZH_OPEN_SWAP reads a file written with the other byte
+ * order, as the reference tools' -swap (Reader_WaveletCompression.h:133-225):
+ * header displacement, metadata, LUTs, record size words and the records'
+ * 4-byte units from byte 4096 on are byte-reversed. */
+#define CZH_OPEN_SWAP 1
+czh_file *czh_open_ex(const char *path, int flags, char *errbuf, size_t errlen);
 void czh_close(czh_file *f);
 int czh_info(const czh_file *f, int *prec, int bpd[3], int sub_bpd[3], int *wtype, double *threshold, int *encoder);
 uint64_t czh_file_bytes(const czh_file *f);

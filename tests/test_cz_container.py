@@ -270,6 +270,95 @@ def test_multi_subdomain_reader(oracle, host, tmp_path):
         assert rec[o + 4:o + 4 + n].tobytes() == pays[g].tobytes()
 
 
+OCEAN = b"\n==============START-BINARY-OCEAN==============\n"
+META = b"==============START-BINARY-METABLOCKS==============\n"
+LUTT = b"\n==============START-BINARY-LUT==============\n"
+
+
+def _swapped_records(raw):
+    """A chunk's records as a writer of the other byte order leaves them for
+    the reference reader's -swap (Reader_WaveletCompression.h:596-620): the
+    size word reversed, and the record's 4-byte units from byte 4096 on."""
+    out, o = bytearray(), 0
+    while o < len(raw):
+        n = struct.unpack_from("<i", raw, o)[0]
+        body = bytearray(raw[o + 4:o + 4 + n])
+        k = 4096 + (n - 4096) // 4 * 4 if n > 4096 else 0
+        if k > 4096:
+            body[4096:k] = np.frombuffer(bytes(body[4096:k]), "<u4").byteswap().tobytes()
+        out += struct.pack(">i", n) + body
+        o += 4 + n
+    return bytes(out)
+
+
+def to_other_byte_order(data, endianess=b"big"):
+    """Rewrite a little-endian .cz as a big-endian writer would have laid it
+    out, field by field from the file layout (SerializerIO_WaveletCompression_
+    MPI_Simple.h:347-466,497-591): every binary size and offset reversed, each
+    chunk's records converted and the chunk re-encoded."""
+    import zlib
+    hd = struct.unpack_from("<Q", data, 0)[0]
+    tail = data[hd:]
+    m = tail.index(META) + len(META)
+    text = tail[:m].decode()
+    kv = dict(l.split(": ", 1) for l in text.splitlines() if ": " in l)
+    bpd = [int(v) for v in kv["Blocks"].split(" x ")]
+    sub = [int(v) for v in kv["SubdomainBlocks"].split(" x ")]
+    enc = kv["Encoder"]
+    nb, nsub = int(np.prod(bpd)), int(np.prod(bpd)) // int(np.prod(sub))
+    meta = np.frombuffer(tail[m:m + 20 * nb], "<i4")
+    h0 = m + 20 * nb + len(LUTT)
+    assert tail[m + 20 * nb:h0] == LUTT
+    oceans, hls, base = [], [], len(OCEAN) + 8
+    for s_ in range(nsub):
+        agg, nc = struct.unpack_from("<Qi", tail, h0 + 12 * s_)
+        lut = list(np.frombuffer(data[base + agg - 8 * nc:base + agg], "<u8")) + [agg - 8 * nc]
+        chunks = []
+        for i in range(nc):
+            c = data[base + lut[i]:base + lut[i + 1]]
+            raw = zlib.decompress(c) if enc == "zlib" else c
+            conv = _swapped_records(raw)
+            chunks.append(zlib.compress(conv) if enc == "zlib" else conv)
+        starts = np.cumsum([0] + [len(c) for c in chunks[:-1]]).astype(">u8")
+        ocean = b"".join(chunks) + starts.tobytes()
+        oceans.append(ocean)
+        hls.append(struct.pack(">Qi", len(ocean), nc))
+        base += agg
+    body = b"".join(oceans)
+    text = text.replace("Endianess: little", "Endianess: " + endianess.decode())
+    return (struct.pack(">Q", 8 + len(OCEAN) + len(body)) + OCEAN + body + text.encode() +
+            meta.astype(">i4").tobytes() + LUTT + b"".join(hls))
+
+
+@pytest.mark.parametrize("kind", ["wt3_zlib_r8", "raw_none", "wt1_none_f64"])
+def test_reader_swap_reads_other_byte_order(oracle, host, tmp_path, kind):
+    """czh_open_ex(CZH_OPEN_SWAP) / CzFile(swap=True) -- the tools' -swap --
+    reads a file of the other byte order to the same records, byte for byte,
+    as the little-endian original; without the flag the file is refused."""
+    if kind == "raw_none":
+        build_case_file(oracle, REF_CZ["raw_f32_r8"], tmp_path / "a.cz")
+    else:
+        d = np.ascontiguousarray(demo("f64" if "f64" in kind else "f32")[:64, :64, :64])
+        s = 8 if "f64" in kind else 4
+        wt = int(kind[2])
+        slots, nb = oracle.compress_field(d, 1e-4, wt)
+        rec, off = czfile.records_from_slots(slots, nb)
+        czfile.write(str(tmp_path / "a.cz"), rec, off, (2, 2, 2), s, wt, 1e-4,
+                     encoder="zlib" if "zlib" in kind else "none",
+                     sub_bpd=(1, 1, 1) if "r8" in kind else (2, 2, 2))
+    good = (tmp_path / "a.cz").read_bytes()
+    with czfile.CzFile(str(tmp_path / "a.cz")) as f:
+        r0, o0 = f.records()
+    for endianess in (b"big", b"little"):  # the reference swaps whatever the header says
+        (tmp_path / "b.cz").write_bytes(to_other_byte_order(good, endianess))
+        with czfile.CzFile(str(tmp_path / "b.cz"), swap=True) as f:
+            r1, o1 = f.records()
+        assert np.array_equal(o1, o0) and np.array_equal(r1, r0)
+        with pytest.raises(czfile.CzFileError):
+            with czfile.CzFile(str(tmp_path / "b.cz")) as f:
+                f.records()
+
+
 def _lz4_cases():
     from tests.golden.make_golden import lz4_generated_inputs
     inputs = lz4_generated_inputs()

@@ -243,3 +243,26 @@ def test_cz2hdf_channels_and_region(demo_raw, tmp_path):
     roi = np.fromfile(tmp_path / "roi.raw", np.float32).reshape(64, 64, 32, 2)
     assert np.array_equal(roi[..., 0], full["a"][32:96, 0:64, 96:128])
     assert np.array_equal(roi[..., 1], full["b"][32:96, 0:64, 96:128])
+
+
+def test_swap_flag_reads_files_of_the_other_byte_order(demo_raw, tmp_path):
+    """cz2diff -swap and cz2hdf -swap (cz2diff.cpp:70-74, cz2hdf.cpp:79,95)
+    on files converted to the other byte order give the same RES line and the
+    same decoded field as the originals without the flag."""
+    from tests.test_cz_container import to_other_byte_order
+    common = ["-rawfile", str(demo_raw), "-bpdx", "4", "-bpdy", "4", "-bpdz", "4", "-fpath", str(tmp_path)]
+    _tool("hdf2cz", *common, "-czfile", "ref.cz", "-nowavelets", "-encoder", "none")
+    _tool("hdf2cz", *common, "-czfile", "tmp.cz", "-threshold", "0.00005", "-wtype", "3")
+    for n in ("ref", "tmp"):
+        (tmp_path / f"{n}_be.cz").write_bytes(to_other_byte_order((tmp_path / f"{n}.cz").read_bytes()))
+    out = _tool("cz2diff", "-czfile1", str(tmp_path / "tmp_be.cz"), "-wtype", "3", "-czfile2",
+                str(tmp_path / "ref_be.cz"), "-swap")
+    assert [l for l in out.splitlines() if l.startswith("RES:")][-1] == PUBLISHED_F32
+    _tool("cz2hdf", "-czfile", str(tmp_path / "tmp.cz"), "-rawfile", str(tmp_path / "a.raw"))
+    _tool("cz2hdf", "-czfile", str(tmp_path / "tmp_be.cz"), "-rawfile", str(tmp_path / "b.raw"), "-swap")
+    assert (tmp_path / "a.raw").read_bytes() == (tmp_path / "b.raw").read_bytes()
+    # without -swap the converted file is refused, not misread
+    exe = os.path.join(BIN, "cz2diff")
+    r = subprocess.run([exe, "-czfile1", str(tmp_path / "tmp_be.cz"), "-wtype", "3", "-rawfile2", str(demo_raw)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
