@@ -64,6 +64,8 @@ def parse():
                     help="skip the eps x zerobits sweep of the 512^3 field, the 128^3 config-0 case and the "
                          "comparison with the f64 path")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg (8 x 128^3)")
+    ap.add_argument("--no-lowmem", action="store_true",
+                    help="skip the leg that times the one-launch compressor framing its own records")
     ap.add_argument("--no-f64", action="store_true", help="skip the binary64 scale-up leg (1024^3 f64)")
     ap.add_argument("--f64-size", type=int, default=1024)
     ap.add_argument("--no-cz", action="store_true", help="skip the end-to-end .cz (host zlib) measurement")
@@ -227,6 +229,56 @@ def batch_leg(a, torch, czw, fields):
             "speedup_vs_plain": round(t_plain / min(t_graph, t_eager), 3),
             "autotuned": {"compress_ms": round(ms_c, 4), "decompress_ms": round(ms_d, 4), "shapes": shape},
             "matches_per_field": same}
+
+
+def lowmem_leg(a, torch, czw, field, out, err, shape, nblocks, rec_ref, off_ref):
+    """The one-launch compressor that frames its own records (record offsets
+    by device look-back; czw_compress_field_lowmem) + the same decoder, over
+    the headline field: step and compress-stage times, its scratch against
+    the two-stage workspace, and its records against the two-stage ones."""
+    prec = 4
+    dev = field.device
+    scratch = torch.empty(czw.lowmem_bytes(nblocks, prec), dtype=torch.uint8, device=dev)
+    records = torch.empty(czw.records_bound(nblocks, prec), dtype=torch.uint8, device=dev)
+    offsets = torch.empty(nblocks + 1, dtype=torch.int64, device=dev)
+
+    def step(ev=None):
+        if ev:
+            ev[0].record()
+        czw.compress_field_lowmem(field, a.eps, a.wtype, a.zerobits, bool(a.shuffle), records, offsets, scratch,
+                                  check=False)
+        if ev:
+            ev[1].record()
+        czw.decompress_field(records, offsets, shape, torch.float32, a.wtype, a.shuffle, out=out, err=err,
+                             check=False)
+        if ev:
+            ev[2].record()
+
+    gc.disable()
+    for _ in range(a.warmup):
+        step()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(a.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / a.steps * 1e-3
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(a.steps)]
+    for k in range(a.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    gc.enable()
+    t_c = sum(e[0].elapsed_time(e[1]) for e in evs) / a.steps * 1e-3
+    if czw.workspace_errors(scratch) or int(err.item()):
+        raise SystemExit("lowmem compressor or decoder reported an error")
+    ne = int(off_ref[-1].item())
+    same = bool(torch.equal(offsets, off_ref) and torch.equal(records[:ne], rec_ref[:ne]))
+    field_bytes = field.numel() * prec
+    return {"GBps": round(field_bytes / t / 1e9, 3), "ms_per_step": round(t * 1e3, 4),
+            "compress_ms": round(t_c * 1e3, 4), "launches_per_step": 2,
+            "scratch_bytes": int(scratch.numel()), "two_stage_workspace_bytes": int(czw.workspace_bytes(nblocks, prec)),
+            "records_equal_two_stage": same}
 
 
 def f64_leg(a, torch, czw, fields):
@@ -588,6 +640,11 @@ def main():
                 torch.cuda.synchronize()
                 sweep_recs[f"eps{eps:g}_zb{zb}"] = (eps, zb, records[: int(offsets[-1].item())].cpu().numpy(),
                                                     offsets.cpu().numpy(), out.cpu().numpy())
+    lowmem = None
+    if not a.no_lowmem:
+        step()
+        torch.cuda.synchronize()
+        lowmem = lowmem_leg(a, torch, czw, field, out, err, shape, nblocks, records, offsets)
     config0 = None if a.no_sweep else config0_leg(a, torch, czw, fields)
     batched = None if a.no_batch else batch_leg(a, torch, czw, fields)
     adjoint = None if a.no_adjoint else adjoint_leg(a, torch, fields)
@@ -726,6 +783,7 @@ def main():
         "other_wtypes": other,
         "eps_zerobits_sweep": sweep or None,
         "config0": {k: v for k, v in config0.items() if k != "_host"} if config0 else None,
+        "fused_framing": lowmem,
         "batched": batched,
         "adjoint": adjoint,
         "f64_scaleup": f64,

@@ -135,8 +135,16 @@ CZW_DI void load_xrows_f32(const float *__restrict__ src, const FieldGeom &g, in
 // its XCDs' L2 caches.
 constexpr uint64_t ST_A = 1ull << 62, ST_P = 2ull << 62, ST_E = 3ull << 62, ST_VAL = (1ull << 62) - 1;
 
-CZW_DI uint64_t st_load(const uint64_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-CZW_DI void st_store(uint64_t *p, uint64_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+// The stores are device-scope atomics (global-memory instructions: a flat
+// access would also hold up the LDS waits).  The reads are plain buffer loads
+// with the device-coherent cache policy (sc1, as the device-scope atomic load
+// is), so that the compiler need not wait for each before issuing the next
+// (a wait on vector memory also waits for every load issued before, here the
+// next block's prefetch); a re-read is kept apart from the read before it by
+// a compiler barrier.
+typedef __attribute__((address_space(1))) uint64_t g_u64;
+constexpr int ST_CPOL = 16;  // sc1
+CZW_DI void st_store(uint64_t *p, uint64_t v) { __hip_atomic_store((g_u64 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 // Record offset of block b (the sum of its predecessors' record bytes), found
 // by one wave from its predecessors' status words, nearest first, down to the
@@ -154,10 +162,13 @@ struct LookBack {
 CZW_DI void look_back_issue(const uint64_t *status, int j, LookBack &lb)
 {
     const int lane = threadIdx.x & 63;
+    const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(status);
+    asm volatile("" ::: "memory");
 #pragma unroll
-    for (int w = 0; w < LB_WIN; ++w) {
+    for (int w = 0; w < LB_WIN; ++w) {  // unconditional loads (word 0 exists), then a select
         const int i = j - lane - 64 * w;
-        lb.s[w] = i >= 0 ? st_load(status + i) : ST_P;  // before block 0: prefix 0
+        const czw_u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(rs, (i >= 0 ? i : 0) * 8, 0, ST_CPOL);
+        lb.s[w] = i >= 0 ? (((uint64_t)v.y << 32) | v.x) : ST_P;  // before block 0: prefix 0
     }
 }
 

@@ -3,8 +3,9 @@ records (record offsets by device look-back) with a small scratch.  Its
 records and offsets must equal czw_compress_field's byte for byte (that path
 is pinned against the oracle in test_gpu_parity.py); one case is checked
 against the oracle here as well.  The look-back's bounded wait is forced to
-give up from the host (a re-read limit of 1), and must end with the error
-bit set and the kernel finished, never a hang."""
+give up from the host (a re-read limit of 1, on a field whose every 16th
+block is dense and slow), and must end with the error bit set and the kernel
+finished, never a hang."""
 import numpy as np
 import pytest
 
@@ -87,15 +88,19 @@ def test_lowmem_refuses_f64_and_small_scratch(torch_gpu):
 def test_lookback_give_up(torch_gpu):
     """With one re-read allowed, blocks whose predecessors are still being
     compressed give up: the call ends (no hang), the error word says so, and
-    the next call with the default limit is correct again."""
+    the next call with the default limit is correct again.  A block is framed
+    two of its workgroup's blocks after it was compressed, by when its
+    predecessors normally have their sizes published; the dense blocks of
+    the first x-slab (every 16th block of the queue, all survivors kept)
+    take longer than that."""
     torch = torch_gpu
-    f = _smooth_plus_noise(torch, (512, 512, 512), 4, 0.0)
+    f = _smooth_plus_noise(torch, (512, 512, 512), 4, 1.0 / 16)
     want = czw.compress_field(f, 1e-3, 1, 0, False)
     scratch = torch.empty(czw.lowmem_bytes(4096, 4), dtype=torch.uint8, device="cuda")
     prev = czw.set_lookback_limit(1)
     errs = 0
     try:
-        for rep in range(3):
+        for rep in range(4):
             print(f"[case] look-back limit 1, run {rep}", flush=True)
             rec, off = czw.compress_field_lowmem(f, 1e-3, 1, 0, False, scratch=scratch, check=False)
             torch.cuda.synchronize()
