@@ -82,18 +82,16 @@ def parse():
 PROFILE = os.path.join(ROOT, "profiles", "czw_f32_field512_eps1e-3_wtype1.md")
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch of `kernel` on the headline workload, from the
-    calibrated FETCH_SIZE/WRITE_SIZE passes recorded in the rocprof summary."""
+def profiled_traffic(role):
+    """HBM bytes per launch of the kernel in `role` ("compress", "decompress",
+    "pack", "compress_lowmem") on the headline workload, from the calibrated
+    FETCH_SIZE/WRITE_SIZE passes recorded in the rocprof summary."""
     if not os.path.exists(PROFILE):
         return None
     for line in open(PROFILE):
         if line.startswith("traffic_bytes_per_launch:"):
-            table = json.loads(line.split(":", 1)[1])
-            key = kernel.split("<")[0]
-            for name, v in table.items():
-                if key in name and (("<float" in name) == ("<float" in kernel)):
-                    return int(v)
+            v = json.loads(line.split(":", 1)[1]).get(role)
+            return int(v) if v is not None else None
     return None
 
 
@@ -671,7 +669,8 @@ def main():
     dom = max(kern, key=lambda k: kern[k][1])
     achieved = kern[dom][0] / kern[dom][1]
     headline = (n == 512 and a.wtype == 1 and a.eps == 1e-3 and a.zerobits == 0 and not a.shuffle and a.bubbles == 200)
-    traffic = profiled_traffic(dom) if headline else None
+    traffic = profiled_traffic({"k_compress_f32": "compress", "k_decompress_h<float>": "decompress"}[dom]) \
+        if headline else None
 
     recs = records[: int(off[-1])].cpu().numpy()
     outh = out.cpu().numpy()
