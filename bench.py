@@ -151,9 +151,43 @@ def _events(torch, n):
     return [torch.cuda.Event(enable_timing=True) for _ in range(n)]
 
 
+def one_field_batch(a, torch, czw, field, single_ms):
+    """The headline field alone as a batch: its compress stage with each tile
+    (whole block, half block; best of the launch shapes the autotuner tries)
+    against the single-field compressor + framing kernel, at the same
+    survivor density (the per-block cost of the batch path)."""
+    from cubismz_amd import batch as B
+
+    bt = B.Batch([field], [a.eps], wtype=a.wtype)
+    res = {}
+    for tile in (B.TILE_BLOCK, B.TILE_HALF):
+        best = None
+        for groups, queue in ((0, 1), (0, 0)):
+            bt.set_launch(groups, queue, stage="compress", tile=tile)
+            for _ in range(3):
+                bt.compress()
+            e = _events(torch, 2)
+            e[0].record()
+            for _ in range(a.steps):
+                bt.compress()
+            e[1].record()
+            torch.cuda.synchronize()
+            ms = e[0].elapsed_time(e[1]) / a.steps
+            best = ms if best is None else min(best, ms)
+        res[B.TILES[tile]] = round(best, 4)
+    ms_tuned = bt.autotune("compress")
+    bt.compress()
+    torch.cuda.synchronize()
+    del bt
+    return {"compress_ms_by_tile": res, "compress_ms_autotuned": round(ms_tuned, 4),
+            "single_field_compress_ms": round(single_ms, 4),
+            "batched_over_single": round(min(res.values()) / single_ms, 3)}
+
+
 def batch_leg(a, torch, czw, fields):
     """64 uneven fields, own seed and eps each: one persistent launch per stage
-    (graph-captured, autotuned launch shape) against 64 plain per-field calls."""
+    (graph-captured, autotuned launch shape and tile) against 64 plain per-field
+    calls."""
     from cubismz_amd import batch as B
 
     shapes, eps = fields.batched_shapes(64, seed=7)
@@ -645,6 +679,8 @@ def main():
         lowmem = lowmem_leg(a, torch, czw, field, out, err, shape, nblocks, records, offsets)
     config0 = None if a.no_sweep else config0_leg(a, torch, czw, fields)
     batched = None if a.no_batch else batch_leg(a, torch, czw, fields)
+    if batched is not None:
+        batched["one_field_batch"] = one_field_batch(a, torch, czw, field, (t_c + t_p) * 1e3)
     adjoint = None if a.no_adjoint else adjoint_leg(a, torch, fields)
     f64 = None if a.no_f64 else f64_leg(a, torch, czw, fields)
     # restore the headline wavelet's records for the checks below

@@ -7,7 +7,10 @@ for that field alone (the per-block compressor is
 WaveletCompressorGeneric::compress, WaveletCompressor.cpp:153-182); the
 batch only changes how blocks are scheduled: one block queue over every
 field, per-field eps from the field table, optional dynamic work queue,
-HIP-graph capture of the whole stage sequence, and a launch-shape autotuner.
+HIP-graph capture of the whole stage sequence, and an autotuner of the launch
+shape (workgroups, queue or static order) and, for the binary32 compress
+stage, of the LDS tile and sweep schedule (TILE_BLOCK or TILE_HALF, see
+czw.h).
 """
 import ctypes as C
 
@@ -20,8 +23,12 @@ FIELD_DTYPE = np.dtype([("src", "<u8"), ("dst", "<u8"), ("nx", "<i4"), ("ny", "<
 MAX_FIELDS = 64
 
 
+TILE_BLOCK, TILE_HALF = 0, 1
+TILES = {TILE_BLOCK: "block", TILE_HALF: "half"}
+
+
 class _Shape(C.Structure):
-    _fields_ = [("groups", C.c_int), ("queue", C.c_int)]
+    _fields_ = [("groups", C.c_int), ("queue", C.c_int), ("tile", C.c_int)]
 
 
 class _Batch(C.Structure):
@@ -132,20 +139,24 @@ class Batch:
         self.err = torch.zeros(1, dtype=torch.int32, device=dev)
         self.desc = _Batch(self.table_dev.data_ptr(), len(self.fields), self.nblocks, self.prec, int(wtype),
                            int(zerobits), int(bool(shuffle)), self.workspace.data_ptr(), self.records.data_ptr(),
-                           self.offsets.data_ptr(), self.records.numel(), self.err.data_ptr(), _Shape(0, 0),
-                           _Shape(0, 0))
+                           self.offsets.data_ptr(), self.records.numel(), self.err.data_ptr(), _Shape(0, 0, 0),
+                           _Shape(0, 0, 0))
 
     @property
     def launch_shape(self):
-        """Launch shape of each stage: {"compress": {"groups", "queue"}, "decompress": {...}}."""
-        return {st: {"groups": getattr(self.desc, st).groups, "queue": getattr(self.desc, st).queue}
+        """Launch shape of each stage: {"compress": {"groups", "queue", "tile"}, "decompress": {...}}
+        (tile "block" or "half"; the decompress stage always runs the half-block tile)."""
+        return {st: {"groups": getattr(self.desc, st).groups, "queue": getattr(self.desc, st).queue,
+                     "tile": TILES[getattr(self.desc, st).tile] if st == "compress" else "half"}
                 for st in STAGES}
 
-    def set_launch(self, groups=0, queue=0, stage=None):
-        """Set the launch shape of one stage, or of both when stage is None."""
+    def set_launch(self, groups=0, queue=0, stage=None, tile=TILE_BLOCK):
+        """Set the launch shape of one stage, or of both when stage is None
+        (tile: the binary32 compress stage's TILE_BLOCK or TILE_HALF)."""
         for st in (STAGES if stage is None else (stage,)):
             sh = getattr(self.desc, st)
             sh.groups, sh.queue = int(groups), int(queue)
+            sh.tile = int(tile) if st == "compress" else 0
 
     def compress(self, check=False):
         """Compress every field into ``records``/``offsets``.  With check, wait

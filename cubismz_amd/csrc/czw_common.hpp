@@ -89,6 +89,16 @@ CZW_DI size_t field_row(const FieldGeom &g, int b, int y, int z, size_t *xstride
     return ((size_t)(32 * ix) * g.ny + (size_t)(32 * iy + y)) * g.nz + (size_t)(32 * iz + z);
 }
 
+// A wave-uniform pointer as scalar registers.
+template <typename T>
+CZW_DI T *uniform_ptr(T *p)
+{
+    const unsigned long long u = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return reinterpret_cast<T *>(((unsigned long long)hi << 32) | lo);
+}
+
 // The block a workgroup works on, resolved from the batch-wide index b: the
 // field it belongs to (source or destination, geometry, threshold) and its
 // index lb inside that field.  Outside LAYOUT_BATCH this is the launch's one
@@ -127,9 +137,13 @@ CZW_DI BlockRef<R> locate(const czw_field_t *T, int nf, int b, const R *src, R *
             if (T[mid].block0 <= b) lo = mid;
             else hi = mid - 1;
         }
+        // the table entry is the same for the whole workgroup: scalar registers,
+        // so that the row addresses and the buffer offsets built from them are too
         const czw_field_t &f = T[lo];
-        return BlockRef<R>{static_cast<const R *>(f.src), static_cast<R *>(f.dst), FieldGeom{f.nx, f.ny, f.nz},
-                           b - f.block0, f.eps};
+        auto u32 = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+        const float e = __int_as_float(u32(__float_as_int(f.eps)));
+        return BlockRef<R>{uniform_ptr(static_cast<const R *>(f.src)), uniform_ptr(static_cast<R *>(f.dst)),
+                           FieldGeom{u32(f.nx), u32(f.ny), u32(f.nz)}, u32(b - f.block0), e};
     }
 }
 
@@ -151,16 +165,6 @@ constexpr int CZW_FIELD_CPOL = 2;  // nt on gfx950 buffer instructions
 inline bool field_fits_buffer(int ny, int nz, int prec)
 {
     return 32ull * (unsigned long long)ny * (unsigned long long)nz * (unsigned long long)prec < (1ull << 31);
-}
-
-// A wave-uniform pointer as scalar registers.
-template <typename T>
-CZW_DI T *uniform_ptr(T *p)
-{
-    const unsigned long long u = reinterpret_cast<unsigned long long>(p);
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
-    return reinterpret_cast<T *>(((unsigned long long)hi << 32) | lo);
 }
 
 // Raw buffer resource over [base, base + 2 GiB) (stride 0, no swizzle).
@@ -331,9 +335,9 @@ inline size_t size_words(int prec) { return prec == 8 ? 9 : 2; }
 // Half-block kernels (czw_half.hip) for Real of prec bytes; layout is
 // LAYOUT_BLOCKS, LAYOUT_FIELD(_WIDE) or LAYOUT_BATCH (then src/dst come from
 // the field table); groups = 0 launches every resident workgroup; queue =
-// nullptr keeps the static block order.  The compressor and packer here are
-// binary64 only (binary32 compresses in czw_kernels.hip); the decoder takes
-// both precisions.
+// nullptr keeps the static block order.  The compressor and packer here
+// serve binary64 (binary32 compresses in czw_kernels.hip, except batches
+// launched with the half-block tile); the decoder takes both precisions.
 int decompress_half(int prec, int layout, const void *records, size_t rec_bytes, const uint64_t *offsets,
                     int nblocks, FieldGeom g, int wtype, int shuffle, void *dst, uint32_t *err,
                     const czw_field_t *fields, int nf, int groups, uint32_t *queue, hipStream_t s);
@@ -342,8 +346,14 @@ int compress_slots_f64(int layout, const void *src, int nblocks, FieldGeom g, fl
                        uint32_t *queue, hipStream_t s);
 int pack_records_f64(const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle, uint8_t *records,
                      uint64_t *offsets, hipStream_t s);
+// Binary32 batches with the half-block tile (nbytes: 9 words per block).
+int compress_slots_h32_batch(const czw_field_t *fields, int nf, int nblocks, int wtype, int zerobits, uint8_t *slots,
+                             uint32_t *nbytes, int groups, uint32_t *queue, hipStream_t s);
+int pack_records_h32(const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle, uint8_t *records,
+                     uint64_t *offsets, hipStream_t s);
 // Resident workgroups (device-wide) of the batch-layout half-block kernels.
 int resident_compress_f64(int layout, int wtype);
+int resident_compress_h32(int wtype);
 int resident_decompress(int prec, int layout, int wtype, int shuffle);
 
 }  // namespace czw

@@ -745,7 +745,8 @@ static int compress_slots_t(int layout, const void *src, int nblocks, FieldGeom 
 }
 
 // The binary32 compressor is k_compress_f32 (czw_kernels.hip); the half-block
-// compressor and its packer are instantiated for binary64 only.
+// compressor and its packer serve binary64, and binary32 batches as the
+// second tile a batch's launch shape can name.
 int compress_slots_f64(int layout, const void *src, int nblocks, FieldGeom g, float eps, int wtype, int zerobits,
                        uint8_t *slots, uint32_t *nbytes, const czw_field_t *fields, int nf, int groups,
                        uint32_t *queue, hipStream_t s)
@@ -835,6 +836,37 @@ int decompress_half(int prec, int layout, const void *records, size_t rec_bytes,
     return CZW_EINVAL;
 }
 
+
+// Binary32 batches with the half-block tile: slots in the same region layout
+// as binary64 (nbytes holds 9 words per block), framed by k_pack_records_h.
+int compress_slots_h32_batch(const czw_field_t *fields, int nf, int nblocks, int wtype, int zerobits, uint8_t *slots,
+                             uint32_t *nbytes, int groups, uint32_t *queue, hipStream_t s)
+{
+    const int wt = wt_index(wtype);
+    if (wt < 0 || !zb_ok(zerobits) || nblocks <= 0 || !slots || !nbytes || !fields) return CZW_EINVAL;
+    const uint32_t zkeep = zerobits ? ~((1u << zerobits) - 1u) : 0xffffffffu;
+    return launch_compress_h<float, LAYOUT_BATCH>(wt, nullptr, FieldGeom{0, 0, 0}, nblocks, 0.0f, zkeep, slots, nbytes,
+                                                  nbytes + nblocks, fields, nf, groups, queue, s);
+}
+
+int pack_records_h32(const uint8_t *slots, const uint32_t *nbytes, int nblocks, int shuffle, uint8_t *records,
+                     uint64_t *offsets, hipStream_t s)
+{
+    if (nblocks <= 0 || !slots || !nbytes || !records || !offsets) return CZW_EINVAL;
+    if (scan_records(nbytes, nblocks, offsets, s) != CZW_OK) return CZW_EHIP;
+    hipLaunchKernelGGL(k_pack_records_h<float>, dim3(nblocks), dim3(256), 0, s, slots, nbytes, nbytes + nblocks,
+                       shuffle, offsets, records);
+    return hipGetLastError() == hipSuccess ? CZW_OK : CZW_EHIP;
+}
+
+int resident_compress_h32(int wtype)
+{
+    const int wt = wt_index(wtype);
+    const void *fn = wt == 1 ? (const void *)k_compress_h<float, 1, LAYOUT_BATCH>
+                   : wt == 2 ? (const void *)k_compress_h<float, 2, LAYOUT_BATCH>
+                             : (const void *)k_compress_h<float, 3, LAYOUT_BATCH>;
+    return resident_groups(fn, NT);
+}
 
 // Resident workgroups of the batch-layout kernels a launch would use.
 int resident_compress_f64(int layout, int wtype)

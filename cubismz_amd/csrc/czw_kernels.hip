@@ -272,6 +272,17 @@ __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ 
     const int t = threadIdx.x;
     const int G = gridDim.x;
     load_table<LAYOUT>(T, fields, nf);
+    // A single HDF5-ordered field goes through a one-entry table as well:
+    // its geometry then comes from LDS into scalar registers when a block
+    // starts, rather than staying in scalar registers through the passes
+    // (which spilled them).  Block b of the launch is block b + b0 of the field.
+    constexpr bool ONE = LAYOUT == LAYOUT_FIELD || LAYOUT == LAYOUT_FIELD_WIDE;
+    constexpr int TL = ONE ? LAYOUT_BATCH : LAYOUT;
+    const int ntab = ONE ? 1 : nf;
+    if constexpr (ONE) {
+        if (t == 0) T[0] = czw_field_t{src, nullptr, g.nx, g.ny, g.nz, eps, -g.b0, 0};
+        __syncthreads();
+    }
     const bool queue = REC || out.queue;  // REC always queues: a block waits only for blocks issued before it
     // slot mode reports no device errors: clear the error word of the header (if given)
     if (!REC && out.hdr && blockIdx.x == 0 && t == 0) out.hdr[1] = 0u;
@@ -286,7 +297,7 @@ __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ 
     float nx[CR][32];
     float eps_b = eps;
     if (b < nblocks) {
-        const BlockRef<float> br = locate<LAYOUT, float>(T, nf, b, src, nullptr, g, eps);
+        const BlockRef<float> br = locate<TL, float>(T, ntab, b, src, nullptr, g, eps);
         eps_b = br.eps;
         load_xrows_f32<RL, NTH, 0, 32>(br.src, br.g, br.lb, t, nx);
     }
@@ -388,11 +399,11 @@ __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ 
         const int bnext = queue ? tk[(it + 1) % 3] : b + G;
         const int bn = min(bnext, nblocks - 1);
         const float eps_c = eps_b;
+        const BlockRef<float> brn = locate<TL, float>(T, ntab, bn, src, nullptr, g, eps);  // found once per block
         auto prefetch = [&](auto x0c) {
             constexpr int X0 = decltype(x0c)::value;
-            const BlockRef<float> br = locate<LAYOUT, float>(T, nf, bn, src, nullptr, g, eps);
-            if constexpr (X0 == 0) eps_b = br.eps;
-            load_xrows_f32<RL, NTH, X0, X0 + 8>(br.src, br.g, br.lb, t, nx);
+            if constexpr (X0 == 0) eps_b = brn.eps;
+            load_xrows_f32<RL, NTH, X0, X0 + 8>(brn.src, brn.g, brn.lb, t, nx);
         };
         prefetch(std::integral_constant<int, 0>());
         {  // level 32, y-pass: rows (z, x) = (q >> 5, q & 31); every row is
@@ -989,12 +1000,32 @@ static bool batch_ok(const czw_batch_t *B)
 {
     return B && B->fields && B->nfields > 0 && B->nfields <= BATCH_MAX_FIELDS && B->nblocks > 0 && B->workspace &&
            B->records && B->offsets && B->compress.groups >= 0 && B->decompress.groups >= 0 &&
-           wt_index(B->wtype) > 0 && (B->prec == 4 || B->prec == 8);
+           (B->compress.tile == CZW_TILE_BLOCK || B->compress.tile == CZW_TILE_HALF) && wt_index(B->wtype) > 0 &&
+           (B->prec == 4 || B->prec == 8);
+}
+
+// A batch's workspace: the header, the slots, then size_words of either
+// tile per block (binary32: 9 for the half-block tile's region counts).
+static size_t batch_workspace_bytes(int nblocks, int prec)
+{
+    if (nblocks <= 0 || (prec != 4 && prec != 8)) return 0;
+    return WS_HDR + (slot_bytes_of(prec) + 4 * size_words(8)) * (size_t)nblocks;
 }
 
 static int batch_compress(const czw_batch_t *B, hipStream_t s)
 {
     if (!batch_ok(B) || !zb_ok(B->zerobits)) return CZW_EINVAL;
+    if (B->prec == 4 && B->compress.tile == CZW_TILE_HALF) {
+        uint32_t *hdr = static_cast<uint32_t *>(B->workspace);
+        if (fill_u32(hdr, (int)(WS_HDR / 4), 0u, s) != CZW_OK) return CZW_EHIP;
+        uint8_t *slots = static_cast<uint8_t *>(B->workspace) + WS_HDR;
+        uint32_t *nb = reinterpret_cast<uint32_t *>(slots + SLOT32 * (size_t)B->nblocks);
+        const int rc = compress_slots_h32_batch(B->fields, B->nfields, B->nblocks, B->wtype, B->zerobits, slots, nb,
+                                                B->compress.groups, B->compress.queue ? hdr : nullptr, s);
+        if (rc != CZW_OK) return rc;
+        return pack_records_h32(slots, nb, B->nblocks, B->shuffle, static_cast<uint8_t *>(B->records), B->offsets,
+                                s);
+    }
     return compress_2stage<LAYOUT_BATCH>(B->prec, nullptr, B->nblocks, FieldGeom{0, 0, 0}, 0.0f, B->wtype,
                                          B->zerobits, B->shuffle, B->records, B->offsets, B->workspace, B->fields,
                                          B->nfields, B->compress, s);
@@ -1012,9 +1043,10 @@ static int batch_decompress(const czw_batch_t *B, hipStream_t s)
 }
 
 // Resident workgroups of the kernel a batch stage launches.
-static int batch_resident(const czw_batch_t *B, int stage)
+static int batch_resident(const czw_batch_t *B, int stage, int tile)
 {
     const int wt = wt_index(B->wtype);
+    if (stage == 1 && B->prec == 4 && tile == CZW_TILE_HALF) return resident_compress_h32(B->wtype);
     if (stage == 1 && B->prec == 4) return resident_groups(compress_f32_fn<LAYOUT_BATCH, false>(wt), CT);
     return stage == 1 ? resident_compress_f64(LAYOUT_BATCH, B->wtype)
                       : resident_decompress(B->prec, LAYOUT_BATCH, B->wtype, B->shuffle);
@@ -1190,7 +1222,7 @@ int czw_batch_prepare(czw_field_t *fields, int nfields, int prec, int *nblocks)
     return CZW_OK;
 }
 
-size_t czw_batch_workspace_bytes(int nblocks, int prec) { return workspace_bytes(nblocks, prec); }
+size_t czw_batch_workspace_bytes(int nblocks, int prec) { return batch_workspace_bytes(nblocks, prec); }
 
 int czw_batch_compress(const czw_batch_t *batch, hipStream_t stream) { return batch_compress(batch, stream); }
 
@@ -1200,10 +1232,20 @@ int czw_batch_autotune(czw_batch_t *batch, int stage, hipStream_t stream, float 
 {
     if (!batch_ok(batch) || (stage != 1 && stage != 2)) return CZW_EINVAL;
     // every resident workgroup of the stage's kernel, or half of them; with
-    // the block queue or in static order
-    const int R = std::max(1, batch_resident(batch, stage));
-    const int shapes[4][2] = {{R, 1}, {std::max(1, R / 2), 1}, {R, 0}, {std::max(1, R / 2), 0}};
-    const int nshapes = 4;
+    // the block queue or in static order; binary32 compress: either tile
+    int shapes[8][3];
+    int nshapes = 0;
+    const int ntiles = (stage == 1 && batch->prec == 4) ? 2 : 1;
+    for (int tile = 0; tile < ntiles; ++tile) {
+        const int R = std::max(1, batch_resident(batch, stage, tile));
+        const int g[4][2] = {{R, 1}, {std::max(1, R / 2), 1}, {R, 0}, {std::max(1, R / 2), 0}};
+        for (int k = 0; k < 4; ++k) {
+            shapes[nshapes][0] = g[k][0];
+            shapes[nshapes][1] = g[k][1];
+            shapes[nshapes][2] = tile;
+            ++nshapes;
+        }
+    }
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return CZW_EHIP;
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -1218,6 +1260,7 @@ int czw_batch_autotune(czw_batch_t *batch, int stage, hipStream_t stream, float 
     for (int k = 0; k < nshapes && rc == CZW_OK; ++k) {
         sh.groups = shapes[k][0];
         sh.queue = shapes[k][1];
+        if (stage == 1) sh.tile = shapes[k][2];
         auto run = [&]() { return stage == 1 ? batch_compress(&trial, stream) : batch_decompress(&trial, stream); };
         if ((rc = run()) != CZW_OK) break;  // warm-up
         (void)hipEventRecord(e0, stream);

@@ -208,7 +208,22 @@ typedef struct czw_field {
 typedef struct czw_shape {
     int groups; /* persistent workgroups; 0 = as many as are resident (never more) */
     int queue;  /* 1: dynamic block queue; 0: static stride */
+    int tile;   /* binary32 compress stage: CZW_TILE_BLOCK or CZW_TILE_HALF;
+                   ignored elsewhere (binary64 compresses, and every stage
+                   decompresses, with the half-block tile) */
 } czw_shape_t;
+
+/* LDS tile and sweep schedule of the binary32 batch compressor.
+ * CZW_TILE_BLOCK: a whole 32^3 block in LDS per workgroup (one workgroup per
+ *   CU), all level-32 passes on it, the next block's rows prefetched into
+ *   registers meanwhile.
+ * CZW_TILE_HALF: one x-half of a block in LDS at a time (two workgroups per
+ *   CU), the x >= 16 half's level-32 passes and survivors first, then the
+ *   x < 16 half's and the [0,16)^3 corner's; no prefetch, the other
+ *   workgroup on the CU fills the wait for the loads.
+ * Both give the same records. */
+#define CZW_TILE_BLOCK 0
+#define CZW_TILE_HALF 1
 
 typedef struct czw_batch {
     const czw_field_t *fields; /* DEVICE copy of the prepared table */
@@ -230,7 +245,8 @@ typedef struct czw_batch {
 int czw_batch_prepare(czw_field_t *fields, int nfields, int prec, int *nblocks);
 
 /* Scratch of a batch: the compressor workspace of its nblocks (header word 0
- * the compress queue, word 1 the error bits, word 2 the decompress queue). */
+ * the compress queue, word 1 the error bits, word 2 the decompress queue),
+ * with room for either tile's per-block size words. */
 size_t czw_batch_workspace_bytes(int nblocks, int prec);
 
 /* Compress every field (fwt, thresholds, survivor streams, records). */
@@ -239,11 +255,12 @@ int czw_batch_compress(const czw_batch_t *batch, hipStream_t stream);
 /* Decompress every field from batch->records into each field's dst. */
 int czw_batch_decompress(const czw_batch_t *batch, hipStream_t stream);
 
-/* Time the launch shapes (all or half of the stage kernel's resident
- * workgroups x queue or static order, where the stage has both) of one stage
- * (1 compress, 2 decompress) on this batch and store the fastest in that
- * stage's shape only; *best_ms receives its time.  Runs the stage several
- * times on `stream`. */
+/* Time the launch shapes of one stage (1 compress, 2 decompress) on this
+ * batch -- all or half of the stage kernel's resident workgroups x block
+ * queue or static order, and for the binary32 compress stage x both tiles
+ * (CZW_TILE_BLOCK, CZW_TILE_HALF) -- and store the fastest in that stage's
+ * shape only; *best_ms receives its time.  Runs the stage several times on
+ * `stream`. */
 int czw_batch_autotune(czw_batch_t *batch, int stage, hipStream_t stream, float *best_ms);
 
 /* Capture the stages (bit 0 compress, bit 1 decompress) of a batch into an

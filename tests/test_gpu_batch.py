@@ -33,14 +33,16 @@ def _check_against_per_field(torch, bt, fs, eps, wtype, shuffle, zerobits=0):
 
 
 @pytest.mark.parametrize("wtype,shuffle,zerobits", [(3, 0, 0), (1, 1, 0), (2, 0, 4)])
-@pytest.mark.parametrize("shape", [(0, 0), (0, 1), (3, 1), (5, 0)])
+@pytest.mark.parametrize("shape", [(0, 0, 0), (0, 1, 0), (3, 1, 0), (5, 0, 0), (0, 0, 1), (0, 1, 1), (5, 1, 1)])
 def test_batch_equals_per_field(torch_gpu, wtype, shuffle, zerobits, shape):
+    """Both compress tiles (whole block, half block) in every launch shape."""
     torch = torch_gpu
-    groups, queue = shape
-    print(f"[case] batch wtype={wtype} shuffle={shuffle} zb={zerobits} groups={groups} queue={queue}", flush=True)
+    groups, queue, tile = shape
+    print(f"[case] batch wtype={wtype} shuffle={shuffle} zb={zerobits} groups={groups} queue={queue} tile={tile}",
+          flush=True)
     fs = _fields(torch)
     bt = B.Batch(fs, EPS, wtype=wtype, zerobits=zerobits, shuffle=shuffle)
-    bt.set_launch(groups, queue)
+    bt.set_launch(groups, queue, tile=tile)
     bt.compress()
     bt.decompress()
     _check_against_per_field(torch, bt, fs, EPS, wtype, shuffle, zerobits)
@@ -66,11 +68,13 @@ def test_batch_field_against_oracle(torch_gpu, oracle):
     assert np.array_equal(bt.outputs[k].cpu().numpy().view(np.uint32), ref.view(np.uint32))
 
 
-def test_batch_graph_replays(torch_gpu):
+@pytest.mark.parametrize("tile", [B.TILE_BLOCK, B.TILE_HALF])
+def test_batch_graph_replays(torch_gpu, tile):
     torch = torch_gpu
+    print(f"[case] batch graph tile={tile}", flush=True)
     fs = _fields(torch, seed0=30)
     bt = B.Batch(fs, EPS, wtype=1)
-    bt.set_launch(0, 1)
+    bt.set_launch(0, 1, tile=tile)
     bt.compress()
     bt.decompress()
     torch.cuda.synchronize()
@@ -104,6 +108,7 @@ def test_batch_autotune_keeps_results(torch_gpu):
     ms_d = bt.autotune("decompress")
     assert ms_c > 0 and ms_d > 0
     assert shape_c["groups"] > 0 and bt.launch_shape["decompress"]["groups"] > 0
+    assert shape_c["tile"] in ("block", "half")
     # tuning one stage leaves the other stage's shape as it was
     assert bt.launch_shape["compress"] == shape_c
     bt.compress()
@@ -137,12 +142,13 @@ def test_random_batches(torch_gpu, case):
     eps = [float(10 ** rng.uniform(-5, -2)) for _ in range(nf)]
     wtype, zerobits, shuffle = int(rng.integers(1, 4)), int(rng.choice([0, 4])), bool(rng.random() < 0.5)
     groups, queue = int(rng.choice([0, 1, 7])), int(rng.integers(0, 2))
+    tile = int(rng.integers(0, 2))
     print(f"[case] random batch {case}: {nf} fields {dtype} wtype={wtype} zb={zerobits} sh={shuffle} "
-          f"groups={groups} queue={queue}", flush=True)
+          f"groups={groups} queue={queue} tile={tile}", flush=True)
     fs = [fields.bubble_field(s, seed=900 + 31 * case + k, nbubbles=int(rng.integers(1, 8)), dtype=dtype,
                               device="cuda") for k, s in enumerate(shapes)]
     bt = B.Batch(fs, eps, wtype=wtype, zerobits=zerobits, shuffle=shuffle)
-    bt.set_launch(groups, queue)
+    bt.set_launch(groups, queue, tile=tile)
     bt.compress()
     bt.decompress(check=True)
     torch.cuda.synchronize()
@@ -155,16 +161,18 @@ def test_random_batches(torch_gpu, case):
         assert torch.equal(bt.outputs[k].view(iv), dec.view(iv)), k
 
 
-def test_batch_64_fields_of_the_benchmark_shapes(torch_gpu, oracle):
+@pytest.mark.parametrize("tile", [B.TILE_BLOCK, B.TILE_HALF])
+def test_batch_64_fields_of_the_benchmark_shapes(torch_gpu, oracle, tile):
     """A full table: 64 fields with edges from {64, ..., 384} and per-field
     eps log-uniform in [1e-4, 1e-2] (the batched benchmark's generator), every
     field equal to its per-field calls, one small field against the oracle."""
     torch = torch_gpu
     shapes, eps = fields.batched_shapes(64, seed=7)
-    print(f"[case] 64-field batch, {sum(int(np.prod(s)) for s in shapes) // 32768} blocks", flush=True)
+    print(f"[case] 64-field batch, {sum(int(np.prod(s)) for s in shapes) // 32768} blocks, tile={tile}", flush=True)
     fs = [fields.bubble_field(s, seed=100 + k, nbubbles=fields.bubbles_for(s), dtype=torch.float32, device="cuda")
           for k, s in enumerate(shapes)]
     bt = B.Batch(fs, [float(e) for e in eps], wtype=1)
+    bt.set_launch(0, 1, stage="compress", tile=tile)
     bt.compress()
     bt.decompress()
     _check_against_per_field(torch, bt, fs, [float(e) for e in eps], 1, False)
@@ -179,18 +187,18 @@ def test_batch_64_fields_of_the_benchmark_shapes(torch_gpu, oracle):
     assert np.array_equal(bt.outputs[k].cpu().numpy().view(np.uint32), ref.view(np.uint32))
 
 
-@pytest.mark.parametrize("dtype_name", ["float32", "float64"])
+@pytest.mark.parametrize("dtype_name,tile", [("float32", 0), ("float32", 1), ("float64", 0)])
 @pytest.mark.parametrize("queue", [0, 1])
-def test_batch_more_groups_than_resident(torch_gpu, dtype_name, queue):
+def test_batch_more_groups_than_resident(torch_gpu, dtype_name, tile, queue):
     """A launch shape far above the resident workgroups (4096) is clamped to
-    them, for both stages, in static and queue order."""
+    them, for both stages and both binary32 tiles, in static and queue order."""
     torch = torch_gpu
     dtype = getattr(torch, dtype_name)
     shapes = SHAPES[:5]
     fs = [fields.bubble_field(s, seed=300 + k, nbubbles=5, dtype=dtype, device="cuda") for k, s in enumerate(shapes)]
     bt = B.Batch(fs, EPS[:5], wtype=3, shuffle=True)
-    bt.set_launch(4096, queue)
-    print(f"[case] batch {dtype_name} groups=4096 queue={queue}", flush=True)
+    bt.set_launch(4096, queue, tile=tile)
+    print(f"[case] batch {dtype_name} groups=4096 queue={queue} tile={tile}", flush=True)
     bt.compress()
     bt.decompress()
     torch.cuda.synchronize()
