@@ -147,6 +147,29 @@ CZW_DI BlockRef<R> locate(const czw_field_t *T, int nf, int b, const R *src, R *
     }
 }
 
+// A single HDF5-ordered field goes through a one-entry table as well: its
+// geometry then comes from LDS into scalar registers when a block starts,
+// rather than occupying scalar registers through the passes (which spilled
+// them).  Block b of the launch is block b + g.b0 of the field.
+template <int LAYOUT>
+constexpr bool one_field()
+{
+    return LAYOUT == LAYOUT_FIELD || LAYOUT == LAYOUT_FIELD_WIDE;
+}
+template <int LAYOUT>
+constexpr int table_layout()
+{
+    return one_field<LAYOUT>() ? LAYOUT_BATCH : LAYOUT;
+}
+template <int LAYOUT>
+CZW_DI void load_one(czw_field_t *T, const void *src, void *dst, const FieldGeom &g, float eps)
+{
+    if constexpr (one_field<LAYOUT>()) {
+        if (threadIdx.x == 0) T[0] = czw_field_t{src, dst, g.nx, g.ny, g.nz, eps, -g.b0, 0};
+        __syncthreads();
+    }
+}
+
 // Load/store layout of a block's rows: batch fields are HDF5-ordered fields.
 template <int LAYOUT>
 constexpr int row_layout()

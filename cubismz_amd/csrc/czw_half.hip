@@ -209,6 +209,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_compress_h(const
     const int t = threadIdx.x;
     const int G = gridDim.x;
     load_table<LAYOUT>(T, fields, nf);
+    load_one<LAYOUT>(T, src, nullptr, g, eps);
     // pass-row owners: z-pass row (y, x & 15) = (t >> 4, t & 15); y-pass row
     // (z, x & 15) = (yz, t & 15) with yz = (t >> 5) + 16*((t >> 4) & 1), so
     // that a half-wave's 32 lanes take z and z + 16 (16 banks apart, as SPD
@@ -217,7 +218,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_compress_h(const
     const int xh = t & 15, sl = t >> 4, yz = (t >> 5) + 16 * ((t >> 4) & 1);
     for (int b = blockIdx.x, it = 0; b < nblocks; ++it) {
         if (queue && t == 0) ticket[it & 1] = G + (int)atomicAdd(queue, 1u);
-        const BlockRef<R> br = locate<LAYOUT, R>(T, nf, b, src, nullptr, g, eps);
+        const BlockRef<R> br = locate<table_layout<LAYOUT>(), R>(T, one_field<LAYOUT>() ? 1 : nf, b, src, nullptr, g, eps);
         const R e = (R)br.eps;  // fabs(c) > threshold in the coefficient type (FullWaveletTransform.h:84)
         uint8_t *slot = slots + (size_t)b * slot_of<R>();
         R *reg = reinterpret_cast<R *>(slot + MASK_BYTES);
@@ -512,6 +513,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
     const int t = threadIdx.x;
     const int G = gridDim.x;
     load_table<LAYOUT>(T, fields, nf);
+    load_one<LAYOUT>(T, nullptr, dst, g, 0.0f);
     // z-pass rows (y, x & 15) = (t >> 4, t & 15); y-pass rows (yz, t & 15), see k_compress_h
     const int xh = t & 15, sl = t >> 4, yz = (t >> 5) + 16 * ((t >> 4) & 1);
     int bn = 0;
@@ -687,7 +689,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
         }
         __syncthreads();
         // level-32 x-pass on whole rows in registers, then the store
-        const BlockRef<R> br = locate<LAYOUT, R>(T, nf, b, nullptr, dst, g, 0.0f);
+        const BlockRef<R> br = locate<table_layout<LAYOUT>(), R>(T, one_field<LAYOUT>() ? 1 : nf, b, nullptr, dst, g, 0.0f);
 #pragma unroll
         for (int k = 0; k < NROW; ++k) {
             const int q = dec_xrow<RL>(t, k);

@@ -272,17 +272,9 @@ __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ 
     const int t = threadIdx.x;
     const int G = gridDim.x;
     load_table<LAYOUT>(T, fields, nf);
-    // A single HDF5-ordered field goes through a one-entry table as well:
-    // its geometry then comes from LDS into scalar registers when a block
-    // starts, rather than staying in scalar registers through the passes
-    // (which spilled them).  Block b of the launch is block b + b0 of the field.
-    constexpr bool ONE = LAYOUT == LAYOUT_FIELD || LAYOUT == LAYOUT_FIELD_WIDE;
-    constexpr int TL = ONE ? LAYOUT_BATCH : LAYOUT;
-    const int ntab = ONE ? 1 : nf;
-    if constexpr (ONE) {
-        if (t == 0) T[0] = czw_field_t{src, nullptr, g.nx, g.ny, g.nz, eps, -g.b0, 0};
-        __syncthreads();
-    }
+    load_one<LAYOUT>(T, src, nullptr, g, eps);
+    constexpr int TL = table_layout<LAYOUT>();
+    const int ntab = one_field<LAYOUT>() ? 1 : nf;
     const bool queue = REC || out.queue;  // REC always queues: a block waits only for blocks issued before it
     // slot mode reports no device errors: clear the error word of the header (if given)
     if (!REC && out.hdr && blockIdx.x == 0 && t == 0) out.hdr[1] = 0u;
