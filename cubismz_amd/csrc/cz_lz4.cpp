@@ -151,16 +151,7 @@ void compress_impl(const uint8_t *src, size_t n, std::vector<uint8_t> &dst)
                 size_t ml = (size_t)(ip - mstart);
                 if (ml >= 15) {
                     dst[token] += 15;
-                    ml -= 15;
-                    for (; ml > 509; ml -= 510) {
-                        o.put(255);
-                        o.put(255);
-                    }
-                    if (ml > 254) {
-                        ml -= 255;
-                        o.put(255);
-                    }
-                    o.put((uint8_t)ml);
+                    o.lengths(ml - 15);
                 } else {
                     dst[token] += (uint8_t)ml;
                 }
