@@ -39,15 +39,32 @@ CZW_DI void lds_st(R *A, int base, int stride, const R *r)
     for (int k = 0; k < N; ++k) A[base + k * stride] = r[k];
 }
 
-// Wave64 inclusive prefix sum.
+// Wave64 inclusive prefix sum in DPP steps (no LDS round trips): row shifts by
+// 1, 2, 4 and 8 scan each 16-lane row (lanes shifted in from outside the row
+// add 0), then row_bcast:15 adds row 0's total to row 1 and row 2's to row 3,
+// and row_bcast:31 adds lane 31's prefix to rows 2 and 3.
 CZW_DI uint32_t wave_incl_scan(uint32_t x, int lane)
 {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
+    (void)lane;
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1, 3
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2, 3
     return x;
+}
+
+// OR of v over the wave, returned in a scalar register: row shifts by 1, 2,
+// 4 and 8 leave each 16-lane row's OR in its last lane.
+CZW_DI uint32_t wave_or(uint32_t v)
+{
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    return (uint32_t)(__builtin_amdgcn_readlane((int)v, 15) | __builtin_amdgcn_readlane((int)v, 31) |
+                      __builtin_amdgcn_readlane((int)v, 47) | __builtin_amdgcn_readlane((int)v, 63));
 }
 
 // Block-wide exclusive scan of one value per thread.  Returns the exclusive
@@ -203,8 +220,37 @@ typedef float czw_f32x4 __attribute__((ext_vector_type(4)));
 typedef double czw_f64x2 __attribute__((ext_vector_type(2)));
 typedef unsigned czw_u32x4 __attribute__((ext_vector_type(4)));
 
+// Level-16 y-pass row (z, x) of thread t < 256 on the [0,16)^3 corner: the
+// first wave takes exactly the 64 rows z < 8, x < 8 that level 8 also
+// transforms, so the level-8 passes stay inside that wave; the other 192
+// rows go to threads 64..255.
+CZW_DI void corner_yrow(int t, int &z, int &x)
+{
+    if (t < 64) {
+        z = t >> 3;
+        x = t & 7;
+    } else if (t < 128) {
+        z = (t - 64) >> 3;
+        x = 8 + (t & 7);
+    } else {
+        z = 8 + ((t - 128) >> 4);
+        x = t & 15;
+    }
+}
+
+// Ordering point inside one wave between LDS passes whose rows cross lanes:
+// the LDS executes a wave's instructions in issue order, so only the
+// compiler must be kept from moving accesses across it.
+CZW_DI void wave_lds_order()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Remaining forward passes on the [0,16)^3 corner of a block held in LDS at
-// z*SP_ + y*RP_ + x: level 16 (x, y) and level 8 (y fused, z, x).
+// z*SP_ + y*RP_ + x: level 16 (x, y) and level 8 (y fused, z, x).  The
+// level-8 passes run on the first wave alone, behind its level-16 y-rows,
+// with no workgroup barrier between them.
 template <int WT, int RP_, int SP_, typename R>
 CZW_DI void fwt16_8(R *A, int t)
 {
@@ -217,55 +263,63 @@ CZW_DI void fwt16_8(R *A, int t)
     }
     __syncthreads();
     if (t < 256) {  // level 16 y-pass, then level 8 y-pass: row (z, x)
-        const int z = t >> 4, x = t & 15;
+        int z, x;
+        corner_yrow(t, z, x);
         lds_ld<16>(A, z * SP_ + x, RP_, r);
         row<16, true, WT, R>(r);
-        if (z < 8 && x < 8) row<8, true, WT, R>(r);
+        if (t < 64) row<8, true, WT, R>(r);
         lds_st<16>(A, z * SP_ + x, RP_, r);
     }
-    __syncthreads();
-    if (t < 64) {  // level 8 z-pass: row (y, x) in [0,8)^2
-        const int y = t >> 3, x = t & 7;
-        lds_ld<8>(A, y * RP_ + x, SP_, r);
-        row<8, true, WT, R>(r);
-        lds_st<8>(A, y * RP_ + x, SP_, r);
-    }
-    __syncthreads();
-    if (t < 64) {  // level 8 x-pass: row (z, y)
-        const int z = t >> 3, y = t & 7;
-        lds_ld<8>(A, z * SP_ + y * RP_, 1, r);
-        row<8, true, WT, R>(r);
-        lds_st<8>(A, z * SP_ + y * RP_, 1, r);
+    if (t < 64) {
+        wave_lds_order();
+        {  // level 8 z-pass: row (y, x) in [0,8)^2
+            const int y = t >> 3, x = t & 7;
+            lds_ld<8>(A, y * RP_ + x, SP_, r);
+            row<8, true, WT, R>(r);
+            lds_st<8>(A, y * RP_ + x, SP_, r);
+        }
+        wave_lds_order();
+        {  // level 8 x-pass: row (z, y)
+            const int z = t >> 3, y = t & 7;
+            lds_ld<8>(A, z * SP_ + y * RP_, 1, r);
+            row<8, true, WT, R>(r);
+            lds_st<8>(A, z * SP_ + y * RP_, 1, r);
+        }
     }
     __syncthreads();
 }
 
 // Inverse passes of all levels except the final level-32 x-pass, which the
 // caller fuses with the store: level 8 (x, z, y), level 16 (y fused, x, z),
-// level 32 (z fused, y).
+// level 32 (z fused, y).  The first wave runs the level-8 x- and z-passes and
+// then its level-16 y-rows; the other y-rows lie outside [0,8)^3 and start at
+// once.
 template <int WT, int RP_, int SP_, typename R>
 CZW_DI void iwt16_8(R *A, int t)
 {
     {
         R r[16];
-        if (t < 64) {  // level 8 x-pass
-            const int z = t >> 3, y = t & 7;
-            lds_ld<8>(A, z * SP_ + y * RP_, 1, r);
-            row<8, false, WT, R>(r);
-            lds_st<8>(A, z * SP_ + y * RP_, 1, r);
+        if (t < 64) {
+            {  // level 8 x-pass
+                const int z = t >> 3, y = t & 7;
+                lds_ld<8>(A, z * SP_ + y * RP_, 1, r);
+                row<8, false, WT, R>(r);
+                lds_st<8>(A, z * SP_ + y * RP_, 1, r);
+            }
+            wave_lds_order();
+            {  // level 8 z-pass
+                const int y = t >> 3, x = t & 7;
+                lds_ld<8>(A, y * RP_ + x, SP_, r);
+                row<8, false, WT, R>(r);
+                lds_st<8>(A, y * RP_ + x, SP_, r);
+            }
+            wave_lds_order();
         }
-        __syncthreads();
-        if (t < 64) {  // level 8 z-pass
-            const int y = t >> 3, x = t & 7;
-            lds_ld<8>(A, y * RP_ + x, SP_, r);
-            row<8, false, WT, R>(r);
-            lds_st<8>(A, y * RP_ + x, SP_, r);
-        }
-        __syncthreads();
         if (t < 256) {  // level 8 y-pass, then level 16 y-pass
-            const int z = t >> 4, x = t & 15;
+            int z, x;
+            corner_yrow(t, z, x);
             lds_ld<16>(A, z * SP_ + x, RP_, r);
-            if (z < 8 && x < 8) row<8, false, WT, R>(r);
+            if (t < 64) row<8, false, WT, R>(r);
             row<16, false, WT, R>(r);
             lds_st<16>(A, z * SP_ + x, RP_, r);
         }

@@ -487,12 +487,17 @@ __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ 
                 // (v_lshl_add_u32), three VALU per position with the bit
                 // extract.  Both are written as instructions, as the compiler
                 // otherwise rebuilds the addresses from a running index.
+                // Positions z that no row of the wave keeps are skipped on a
+                // scalar test of the wave's OR of the keep bits (most of them
+                // at the densities the threshold leaves).
                 char *Ab = reinterpret_cast<char *>(A);
                 const uint32_t n0 = __popc(kb[k] & 0xffffu), n1 = __popc(kb[k] >> 16);
                 uint32_t a0 = (STAGE32_AT + i0 + n0 - 1) * 4u, a1 = (STAGE32_AT + i1 + n1 - 1) * 4u;
                 const uint32_t ad = (STAGE32_AT + STAGE32 + (t & 63)) * 4u;
+                const uint32_t wk = wave_or(kb[k]);
 #pragma unroll
                 for (int z = 31; z >= 0; --z) {
+                    if (!((wk >> z) & 1u)) continue;
                     const uint32_t m = (uint32_t)((int32_t)(kb[k] << (31 - z)) >> 31);
                     uint32_t &a = z < 16 ? a0 : a1;
                     uint32_t at;
@@ -534,27 +539,30 @@ __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ 
             }
             __syncthreads();
             if (t < 256) {  // level 16 y-pass, then level 8 y-pass: row (z, x)
-                const int z = t >> 4, xx = t & 15;
+                int z, xx;
+                corner_yrow(t, z, xx);
                 lds_ld<16>(A, z * SP + xx, RP, c);
                 row<16, true, WT>(c);
-                if (z < 8 && xx < 8) row<8, true, WT>(c);
+                if (t < 64) row<8, true, WT>(c);
                 lds_st<16>(A, z * SP + xx, RP, c);
             }
-            __syncthreads();
-            if (t < 64) {  // level 8 z-pass: row (y, x) in [0,8)^2
-                const int yy = t >> 3, xx = t & 7;
-                lds_ld<8>(A, yy * RP + xx, SP, c);
-                row<8, true, WT>(c);
-                lds_st<8>(A, yy * RP + xx, SP, c);
+            if (t < 64) {  // the level-8 passes stay in the first wave: no workgroup barrier
+                wave_lds_order();
+                {  // level 8 z-pass: row (y, x) in [0,8)^2
+                    const int yy = t >> 3, xx = t & 7;
+                    lds_ld<8>(A, yy * RP + xx, SP, c);
+                    row<8, true, WT>(c);
+                    lds_st<8>(A, yy * RP + xx, SP, c);
+                }
+                wave_lds_order();
+                {  // level 8 x-pass: row (z, y)
+                    const int z = t >> 3, yy = t & 7;
+                    lds_ld<8>(A, z * SP + yy * RP, 1, c);
+                    row<8, true, WT>(c);
+                    lds_st<8>(A, z * SP + yy * RP, 1, c);
+                }
             } else if (t < 256) {
                 prefetch(std::integral_constant<int, 24>());
-            }
-            __syncthreads();
-            if (t < 64) {  // level 8 x-pass: row (z, y)
-                const int z = t >> 3, yy = t & 7;
-                lds_ld<8>(A, z * SP + yy * RP, 1, c);
-                row<8, true, WT>(c);
-                lds_st<8>(A, z * SP + yy * RP, 1, c);
             }
             __syncthreads();
             if (t < 64) prefetch(std::integral_constant<int, 24>());
@@ -708,9 +716,9 @@ __global__ __launch_bounds__(PK_T) void k_pack_records(const uint8_t *__restrict
 #pragma unroll
             for (int k = 0; k < 8; ++k) part += q[k];
         }
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);
-        if ((t & 63) == 0) wsum[t >> 6] = part;
+        // the sizes of at most PK_FUSED_MAX records sum to less than 2^32
+        const uint32_t wpart = wave_incl_scan((uint32_t)part, t & 63);
+        if ((t & 63) == 63) wsum[t >> 6] = wpart;
         __syncthreads();
         if (t == 0) {
             uint64_t o = 0;
