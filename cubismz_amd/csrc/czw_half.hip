@@ -24,6 +24,8 @@
 // RPD = 17, SPD = 545 (elements): x-rows for either thread order, y- and
 // z-rows and the corner passes are at most 2-way bank conflicted for both
 // element sizes.
+#include <type_traits>
+
 #include "czw_common.hpp"
 
 namespace czw {
@@ -458,6 +460,23 @@ __global__ __launch_bounds__(256) void k_pack_records_h(const uint8_t *__restric
 // Survivors are read straight from the record (they are at 4-byte offsets;
 // byte planes when shuffled).  SHUF is a template parameter so that the
 // per-position gathers carry no test of it.
+// LDS the binary32 decoder has left for staging a block's level-32
+// survivors: two workgroups per CU (2 x 81920 bytes of the 160 KB) with the
+// 64 KB half, mask and counts.  Binary64 (one workgroup per CU, denser
+// records) reads them from the record.
+template <typename R>
+constexpr int DEC_LS_CAP()
+{
+    return sizeof(R) == 4 ? 1728 : 1;
+}
+
+// v where bit is 1, +0 where it is 0, by a mask (no select on v's value).
+CZW_DI float keep_if(float v, uint32_t bit) { return __uint_as_float(__float_as_uint(v) & (0u - bit)); }
+CZW_DI double keep_if(double v, uint32_t bit)
+{
+    return __longlong_as_double(__double_as_longlong(v) & (long long)(0ull - (unsigned long long)bit));
+}
+
 template <typename R, bool SHUF>
 struct Payload {
     const uint32_t *w;  // unshuffled: survivor i = words W*i .. W*i + W - 1
@@ -506,6 +525,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
     __shared__ R H[32 * SPd];
     __shared__ uint32_t M[1024];
     __shared__ uint16_t cnt[2560];  // corner pieces [0,512), level-32 runs 512 + (code-1)*256 + run
+    __shared__ R Ls[DEC_LS_CAP<R>()];  // a block's level-32 survivors, when they fit
     __shared__ uint32_t wsum[NT / 64];
     __shared__ uint32_t flag;
     __shared__ int ticket[2];
@@ -596,6 +616,20 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
             if (t == 0) atomicOr(err, (uint32_t)CZW_ERR_POPCOUNT);
             continue;
         }
+        // The level-32 survivors [sc0, S) (sc0 = the corner's count) go to LDS
+        // when they fit: loaded here in one coalesced pass, stored behind the
+        // corner gather, so that the z-pass gathers below read LDS instead of
+        // the record.  Otherwise those gathers read the record.
+        const uint32_t sc0 = cnt[512];
+        const uint32_t s32 = S - sc0;
+        const bool staged = DEC_LS_CAP<R>() > 1 && s32 <= (uint32_t)DEC_LS_CAP<R>();
+        constexpr int LS_PER = (DEC_LS_CAP<R>() + NT - 1) / NT;
+        R sv[LS_PER];
+        if (staged) {
+#pragma unroll
+            for (int u = 0; u < LS_PER; ++u)
+                if (t + NT * u < (int)s32) sv[u] = pay.get(sc0 + t + NT * u);
+        }
         {  // corner positions: survivor or +0; all eight loads are issued
             // before the first LDS store, so that they overlap
             uint32_t idx = cnt[t];
@@ -616,8 +650,48 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
                     if (i < h) H[addr + i * stride] = v[q * h + i];
             });
         }
+        if (staged) {
+#pragma unroll
+            for (int u = 0; u < LS_PER; ++u)
+                if (t + NT * u < (int)s32) Ls[t + NT * u] = sv[u];
+        }
         __syncthreads();
         iwt16_8<WT, RPd, SPd>(H, t);  // level 8 (x, z, y), level 16 (y, x)
+        // Survivors z in [z0, 32) of one z-row (mask word mw, clear below z0;
+        // run offsets i0 for z < 16, i1 for z >= 16), or +0.  From the record:
+        // one masked load per kept position.  From LDS: positions that no row
+        // of the wave keeps are skipped on a scalar test; at the others every
+        // lane reads (index from a popcount, clamped into Ls) and the mask bit
+        // selects, so that no read waits for the one before.
+        auto gather = [&](auto from_lds, uint32_t mw, uint32_t i0, uint32_t i1, int z0, R *r) {
+            if constexpr (decltype(from_lds)::value) {
+                const uint32_t wm = wave_or(mw);
+                // the run offsets (LDS reads) are consumed before the first
+                // branch, so that no wait inside one covers the gathers
+                uint32_t j0 = i0 - sc0, j1 = i1 - sc0;
+                asm volatile("" : "+v"(j0), "+v"(j1));
+                R v[32];
+#pragma unroll
+                for (int z = 0; z < 32; ++z) {
+                    if (z < z0 || !((wm >> z) & 1u)) continue;
+                    const uint32_t below = mw & ((1u << z) - 1u) & (z < 16 ? 0xffffu : 0xffff0000u);
+                    const uint32_t i = (z < 16 ? j0 : j1) + (uint32_t)__popc(below);
+                    v[z] = Ls[min(i, (uint32_t)DEC_LS_CAP<R>() - 1u)];
+                }
+#pragma unroll
+                for (int z = 0; z < 32; ++z)  // v[z] counts only where some lane read it
+                    if (z >= z0) r[z] = keep_if(v[z], (mw >> z) & 1u);
+            } else {
+#pragma unroll
+                for (int z = 0; z < 32; ++z) {
+                    if (z < z0) continue;
+                    uint32_t &i = z < 16 ? i0 : i1;
+                    const bool bit = (mw >> z) & 1u;
+                    r[z] = bit ? pay.get(i) : (R)0;
+                    i += bit;
+                }
+            }
+        };
 
         // ---- x < 16: level-16 z-pass on the corner columns, level-32 z- and y-passes
         {
@@ -630,22 +704,13 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
                 row<16, false, WT, R>(c);
 #pragma unroll
                 for (int z = 0; z < 16; ++z) r[z] = c[z];
-                uint32_t i1 = cnt[512 + base];  // code 1
-#pragma unroll
-                for (int z = 16; z < 32; ++z) {
-                    const bool bit = (mw >> z) & 1u;
-                    r[z] = bit ? pay.get(i1) : (R)0;
-                    i1 += bit;
-                }
+                const uint32_t i1 = cnt[512 + base];  // code 1
+                if (staged) gather(std::true_type(), mw, 0u, i1, 16, r);
+                else gather(std::false_type(), mw, 0u, i1, 16, r);
             } else {
-                uint32_t i0 = cnt[512 + 3 * 256 + base], i1 = cnt[512 + 4 * 256 + base];  // codes 4, 5
-#pragma unroll
-                for (int z = 0; z < 32; ++z) {
-                    uint32_t &i = z < 16 ? i0 : i1;
-                    const bool bit = (mw >> z) & 1u;
-                    r[z] = bit ? pay.get(i) : (R)0;
-                    i += bit;
-                }
+                const uint32_t i0 = cnt[512 + 3 * 256 + base], i1 = cnt[512 + 4 * 256 + base];  // codes 4, 5
+                if (staged) gather(std::true_type(), mw, i0, i1, 0, r);
+                else gather(std::false_type(), mw, i0, i1, 0, r);
             }
             row<32, false, WT, R>(r);
             lds_st<32>(H, y * RPd + xh, SPd, r);
@@ -668,15 +733,10 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
             const int y = sl, x = 16 + xh, base = (y & 15) * 16 + xh;
             const int c0 = 2 + 4 * (y >> 4);
             const uint32_t mw = M[x + 32 * y];
-            uint32_t i0 = cnt[512 + (c0 - 1) * 256 + base], i1 = cnt[512 + c0 * 256 + base];
+            const uint32_t i0 = cnt[512 + (c0 - 1) * 256 + base], i1 = cnt[512 + c0 * 256 + base];
             R r[32];
-#pragma unroll
-            for (int z = 0; z < 32; ++z) {
-                uint32_t &i = z < 16 ? i0 : i1;
-                const bool bit = (mw >> z) & 1u;
-                r[z] = bit ? pay.get(i) : (R)0;
-                i += bit;
-            }
+            if (staged) gather(std::true_type(), mw, i0, i1, 0, r);
+            else gather(std::false_type(), mw, i0, i1, 0, r);
             row<32, false, WT, R>(r);
             lds_st<32>(H, y * RPd + xh, SPd, r);
         }
