@@ -462,12 +462,12 @@ __global__ __launch_bounds__(256) void k_pack_records_h(const uint8_t *__restric
 // per-position gathers carry no test of it.
 // LDS the binary32 decoder has left for staging a block's level-32
 // survivors: two workgroups per CU (2 x 81920 bytes of the 160 KB) with the
-// 64 KB half, mask and counts.  Binary64 (one workgroup per CU, denser
-// records) reads them from the record.
-template <typename R>
+// 64 KB half, mask and counts (and a batch's 64-entry field table).
+// Binary64 (one workgroup per CU, denser records) reads them from the record.
+template <typename R, int LAYOUT>
 constexpr int DEC_LS_CAP()
 {
-    return sizeof(R) == 4 ? 1728 : 1;
+    return sizeof(R) != 4 ? 1 : LAYOUT == LAYOUT_BATCH ? 1088 : 1728;
 }
 
 // v where bit is 1, +0 where it is 0, by a mask (no select on v's value).
@@ -525,7 +525,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
     __shared__ R H[32 * SPd];
     __shared__ uint32_t M[1024];
     __shared__ uint16_t cnt[2560];  // corner pieces [0,512), level-32 runs 512 + (code-1)*256 + run
-    __shared__ R Ls[DEC_LS_CAP<R>()];  // a block's level-32 survivors, when they fit
+    __shared__ R Ls[DEC_LS_CAP<R, LAYOUT>()];  // a block's level-32 survivors, when they fit
     __shared__ uint32_t wsum[NT / 64];
     __shared__ uint32_t flag;
     __shared__ int ticket[2];
@@ -622,8 +622,8 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
         // the record.  Otherwise those gathers read the record.
         const uint32_t sc0 = cnt[512];
         const uint32_t s32 = S - sc0;
-        const bool staged = DEC_LS_CAP<R>() > 1 && s32 <= (uint32_t)DEC_LS_CAP<R>();
-        constexpr int LS_PER = (DEC_LS_CAP<R>() + NT - 1) / NT;
+        const bool staged = DEC_LS_CAP<R, LAYOUT>() > 1 && s32 <= (uint32_t)DEC_LS_CAP<R, LAYOUT>();
+        constexpr int LS_PER = (DEC_LS_CAP<R, LAYOUT>() + NT - 1) / NT;
         R sv[LS_PER];
         if (staged) {
 #pragma unroll
@@ -676,7 +676,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
                     if (z < z0 || !((wm >> z) & 1u)) continue;
                     const uint32_t below = mw & ((1u << z) - 1u) & (z < 16 ? 0xffffu : 0xffff0000u);
                     const uint32_t i = (z < 16 ? j0 : j1) + (uint32_t)__popc(below);
-                    v[z] = Ls[min(i, (uint32_t)DEC_LS_CAP<R>() - 1u)];
+                    v[z] = Ls[min(i, (uint32_t)DEC_LS_CAP<R, LAYOUT>() - 1u)];
                 }
 #pragma unroll
                 for (int z = 0; z < 32; ++z)  // v[z] counts only where some lane read it
