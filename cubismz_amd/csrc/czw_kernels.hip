@@ -673,7 +673,7 @@ int scan_records(const uint32_t *nbytes, int n, uint64_t *offsets, hipStream_t s
 // offsets itself (sum of the sizes before it: O(nblocks^2) cached reads in
 // all, used for up to PK_FUSED_MAX blocks) and writes offsets[]; otherwise
 // offsets[] comes from k_scan_records.
-constexpr int PK_T = 512, PK_W = 4, PK_B = 2, PK_FUSED_MAX = 4096;
+constexpr int PK_T = 512, PK_W = 8, PK_B = 2, PK_FUSED_MAX = 4096;
 __global__ __launch_bounds__(PK_T) void k_pack_records(const uint8_t *__restrict__ slots, size_t slot_bytes,
                                                        const uint32_t *__restrict__ nbytes, int nblocks, int shuffle,
                                                        int fused, uint64_t *__restrict__ offsets,
