@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "../../include/czw.h"
 #include "czw_wavelet.hpp"
 
@@ -48,11 +50,13 @@ __global__ __launch_bounds__(256) void k_rowpass(R *__restrict__ blocks, int nbl
     for (int k = 0; k < N; ++k) p[k * stride] = v[k];
 }
 
-// Binary32: one workgroup per block.  The block is read once into LDS
-// (O[z][y][x] at z*TSP + y*TRP + x, the padding that makes rows along x, y
-// and z conflict-free), all nine passes of the schedule run there with the
-// same row functions and order as k_rowpass, and the block is written back
-// once: 256 KB of HBM traffic per block instead of nine times that.
+// Binary32: a persistent grid, one block per workgroup at a time.  The block
+// is read once into LDS (O[z][y][x] at z*TSP + y*TRP + x, the padding that
+// makes rows along x, y and z conflict-free), all nine passes of the
+// schedule run there with the same row functions and order as k_rowpass,
+// and the block is written back once: 256 KB of HBM traffic per block
+// instead of nine times that.  The workgroup's next block is loaded into
+// registers while the current one is transformed.
 constexpr int TRP = 33, TSP = 32 * 33 + 1, TT = 512;
 
 template <int WT, int OP, int N>
@@ -86,18 +90,10 @@ CZW_DI void lds_pass(float *A, int t, int ax)
     __syncthreads();
 }
 
+// The nine passes of one block in A (ends with a barrier).
 template <int WT, int OP>
-__global__ __launch_bounds__(TT) void k_transform_lds_f32(float *__restrict__ blocks)
+CZW_DI void transform_block(float *A, int t)
 {
-    __shared__ float A[32 * TSP];
-    const int t = threadIdx.x;
-    float *g = blocks + (size_t)blockIdx.x * 32768;
-#pragma unroll 8
-    for (int k = 0; k < 32768 / TT; ++k) {
-        const int e = t + TT * k;
-        A[(e >> 10) * TSP + ((e >> 5) & 31) * TRP + (e & 31)] = g[e];
-    }
-    __syncthreads();
     enum { Z = 0, Y = 1, X = 2 };
     if constexpr (OP == OP_FWD) {  // level 32: x, y, z; 16: z, x, y; 8: y, z, x
         lds_pass<WT, OP, 32>(A, t, X); lds_pass<WT, OP, 32>(A, t, Y); lds_pass<WT, OP, 32>(A, t, Z);
@@ -112,20 +108,64 @@ __global__ __launch_bounds__(TT) void k_transform_lds_f32(float *__restrict__ bl
         lds_pass<WT, OP, 16>(A, t, Z); lds_pass<WT, OP, 16>(A, t, X); lds_pass<WT, OP, 16>(A, t, Y);
         lds_pass<WT, OP, 8>(A, t, Y); lds_pass<WT, OP, 8>(A, t, Z); lds_pass<WT, OP, 8>(A, t, X);
     }
+}
+
+template <int WT, int OP>
+__global__ __launch_bounds__(TT) void k_transform_lds_f32(float *__restrict__ blocks, int nblocks)
+{
+    __shared__ float A[32 * TSP];
+    const int t = threadIdx.x;
+    constexpr int PER = 32768 / TT;
+    float nxt[PER];
+    auto load = [&](int bb) {
+        const float *g = blocks + (size_t)bb * 32768;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) nxt[k] = g[t + TT * k];
+    };
+    if ((int)blockIdx.x < nblocks) load(blockIdx.x);
+    for (int b = blockIdx.x; b < nblocks; b += gridDim.x) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int e = t + TT * k;
+            A[(e >> 10) * TSP + ((e >> 5) & 31) * TRP + (e & 31)] = nxt[k];
+        }
+        __syncthreads();
+        if (b + (int)gridDim.x < nblocks) load(b + gridDim.x);
+        transform_block<WT, OP>(A, t);
+        float *g = blocks + (size_t)b * 32768;
 #pragma unroll 8
-    for (int k = 0; k < 32768 / TT; ++k) {
-        const int e = t + TT * k;
-        g[e] = A[(e >> 10) * TSP + ((e >> 5) & 31) * TRP + (e & 31)];
+        for (int k = 0; k < PER; ++k) {
+            const int e = t + TT * k;
+            g[e] = A[(e >> 10) * TSP + ((e >> 5) & 31) * TRP + (e & 31)];
+        }
+        __syncthreads();  // A is read before the next block is written
     }
+}
+
+// Workgroups of k_transform_lds_f32 resident on the current device (its LDS
+// block allows one per CU).
+static int transform_groups(const void *fn)
+{
+    static int res[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (res[dev] == 0) {
+        int cus = 0, per = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, TT, 0) != hipSuccess || per <= 0) per = 1;
+        res[dev] = cus * per;
+    }
+    return res[dev];
 }
 
 template <int WT>
 static void transform_lds_f32(float *blocks, int nblocks, int op, hipStream_t s)
 {
-    const dim3 grid(nblocks), blk(TT);
-    if (op == OP_FWD) hipLaunchKernelGGL((k_transform_lds_f32<WT, OP_FWD>), grid, blk, 0, s, blocks);
-    if (op == OP_INV) hipLaunchKernelGGL((k_transform_lds_f32<WT, OP_INV>), grid, blk, 0, s, blocks);
-    if (op == OP_INV_ADJ) hipLaunchKernelGGL((k_transform_lds_f32<WT, OP_INV_ADJ>), grid, blk, 0, s, blocks);
+    const void *fn = (const void *)k_transform_lds_f32<WT, OP_FWD>;
+    const dim3 grid(std::min(nblocks, transform_groups(fn))), blk(TT);
+    if (op == OP_FWD) hipLaunchKernelGGL((k_transform_lds_f32<WT, OP_FWD>), grid, blk, 0, s, blocks, nblocks);
+    if (op == OP_INV) hipLaunchKernelGGL((k_transform_lds_f32<WT, OP_INV>), grid, blk, 0, s, blocks, nblocks);
+    if (op == OP_INV_ADJ) hipLaunchKernelGGL((k_transform_lds_f32<WT, OP_INV_ADJ>), grid, blk, 0, s, blocks, nblocks);
 }
 
 template <typename R, int WT, int OP>
