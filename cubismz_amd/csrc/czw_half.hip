@@ -670,16 +670,20 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_decompress_h(con
                 // branch, so that no wait inside one covers the gathers
                 uint32_t j0 = i0 - sc0, j1 = i1 - sc0;
                 asm volatile("" : "+v"(j0), "+v"(j1));
+                // v[z] starts as an arbitrary but fixed value (a frozen
+                // undefined one, no instruction): positions that the wave
+                // skips are then well defined and the mask bit clears them
                 R v[32];
 #pragma unroll
                 for (int z = 0; z < 32; ++z) {
+                    v[z] = __builtin_nondeterministic_value((R)0);
                     if (z < z0 || !((wm >> z) & 1u)) continue;
                     const uint32_t below = mw & ((1u << z) - 1u) & (z < 16 ? 0xffffu : 0xffff0000u);
                     const uint32_t i = (z < 16 ? j0 : j1) + (uint32_t)__popc(below);
                     v[z] = Ls[min(i, (uint32_t)DEC_LS_CAP<R, LAYOUT>() - 1u)];
                 }
 #pragma unroll
-                for (int z = 0; z < 32; ++z)  // v[z] counts only where some lane read it
+                for (int z = 0; z < 32; ++z)
                     if (z >= z0) r[z] = keep_if(v[z], (mw >> z) & 1u);
             } else {
 #pragma unroll
