@@ -408,7 +408,7 @@ def f64_leg(a, torch, czw, fields):
             # end to end to .cz bytes: host zlib (OpenMP) overlapped slab by slab with the device
             from cubismz_amd import czfile
 
-            threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+            threads = a.cpu_threads or len(os.sched_getaffinity(0))
             fb, tm = czfile.compress_device_field(None, field, eps, a.wtype, nthreads=threads, overlap=True)
             r["cz_file_end_to_end"] = {"file_bytes": fb, "ratio": round(field_bytes / fb, 3), "threads": threads,
                                        "GBps": round(field_bytes / tm["wall"] / 1e9, 3),
