@@ -361,6 +361,9 @@ __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ 
                     }
                 }
             } else {
+                // the corner survivors were staged after the block's last
+                // barrier (REC has the look-back's barrier above)
+                __syncthreads();
                 uint32_t *slot = reinterpret_cast<uint32_t *>(out.dst + (size_t)bp * SLOT32);
 #pragma unroll
                 for (int k = 0; k < CR; ++k) slot[t + NTH * k] = M[t + NTH * k];
