@@ -490,23 +490,29 @@ __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ 
                 // (v_lshl_add_u32), three VALU per position with the bit
                 // extract.  Both are written as instructions, as the compiler
                 // otherwise rebuilds the addresses from a running index.
-                // Positions z that no row of the wave keeps are skipped on a
-                // scalar test of the wave's OR of the keep bits (most of them
-                // at the densities the threshold leaves).
+                // Groups of eight positions z that no row of the wave keeps
+                // are skipped on a scalar test of the wave's OR of the keep
+                // bits (most of them at the densities the threshold leaves);
+                // a kept group runs branch-free (a test per position made
+                // two taken branches per kept position: 2.7% slower).
                 char *Ab = reinterpret_cast<char *>(A);
                 const uint32_t n0 = __popc(kb[k] & 0xffffu), n1 = __popc(kb[k] >> 16);
                 uint32_t a0 = (STAGE32_AT + i0 + n0 - 1) * 4u, a1 = (STAGE32_AT + i1 + n1 - 1) * 4u;
                 const uint32_t ad = (STAGE32_AT + STAGE32 + (t & 63)) * 4u;
                 const uint32_t wk = wave_or(kb[k]);
+                constexpr int C = 8;
 #pragma unroll
-                for (int z = 31; z >= 0; --z) {
-                    if (!((wk >> z) & 1u)) continue;
-                    const uint32_t m = (uint32_t)((int32_t)(kb[k] << (31 - z)) >> 31);
-                    uint32_t &a = z < 16 ? a0 : a1;
-                    uint32_t at;
-                    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(at) : "v"(m), "v"(a), "v"(ad));
-                    *reinterpret_cast<float *>(Ab + at) = r[k][z];
-                    asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(a) : "v"(m), "v"(a));
+                for (int c = 32 / C - 1; c >= 0; --c) {
+                    if (!((wk >> (C * c)) & ((1u << C) - 1u))) continue;
+#pragma unroll
+                    for (int z = C * c + C - 1; z >= C * c; --z) {
+                        const uint32_t m = (uint32_t)((int32_t)(kb[k] << (31 - z)) >> 31);
+                        uint32_t &a = z < 16 ? a0 : a1;
+                        uint32_t at;
+                        asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(at) : "v"(m), "v"(a), "v"(ad));
+                        *reinterpret_cast<float *>(Ab + at) = r[k][z];
+                        asm("v_lshl_add_u32 %0, %1, 2, %2" : "=v"(a) : "v"(m), "v"(a));
+                    }
                 }
             } else {
                 uint32_t *d;
