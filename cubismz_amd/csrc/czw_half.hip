@@ -164,7 +164,31 @@ CZW_DI double absval(double v) { return fabs(v); }
 
 // Stage of the x < 16 half's survivors: the z >= 16 planes of H.
 constexpr int STAGE0_AT = 16 * SPD;
-constexpr int STAGE0 = HALF_WORDS - STAGE0_AT;
+constexpr int STAGE0 = HALF_WORDS - STAGE0_AT - 64;  // 64 dummy words follow
+static_assert(16384 + 64 <= HALF_WORDS, "room for the x >= 16 half's stage and its dummy words");
+
+// One z-row's survivors into an LDS stage st at run offsets i0 (z < 16) and
+// i1 (z >= 16), zerobits applied.  Groups of eight positions that no row of
+// the wave keeps are skipped on a scalar test; a kept group runs branch-free,
+// unkept values going to the lane's dummy word (a test per position costs
+// taken branches).
+template <typename R>
+CZW_DI void stage_row(R *st, R *dummy, uint32_t bits, uint32_t i0, uint32_t i1, const R *r, uint32_t zkeep)
+{
+    const uint32_t wk = wave_or(bits);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        if (!((wk >> (8 * c)) & 0xffu)) continue;
+#pragma unroll
+        for (int z = 8 * c; z < 8 * c + 8; ++z) {
+            const uint32_t bit = (bits >> z) & 1u;
+            uint32_t &i = z < 16 ? i0 : i1;
+            R *p = bit ? st + i : dummy;
+            *p = zero_low_bits(r[z], zkeep);
+            i += bit;
+        }
+    }
+}
 
 // n staged values (LDS, 16-byte aligned) to global memory (16-byte aligned),
 // 16 bytes per store where possible.
@@ -268,12 +292,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_compress_h(const
             }
             // every column is in registers: H becomes the stage of this half's
             // survivors (at most 16384), copied out with 16-byte stores
-            uint32_t i0 = cnt[ri * 256 + base], i1 = cnt[(ri + 1) * 256 + base];
-#pragma unroll
-            for (int z = 0; z < 32; ++z) {
-                uint32_t &i = z < 16 ? i0 : i1;
-                if ((bits >> z) & 1u) H[i++] = zero_low_bits(r[z], zkeep);
-            }
+            stage_row<R>(H, H + 16384 + (t & 63), bits, cnt[ri * 256 + base], cnt[(ri + 1) * 256 + base], r, zkeep);
         }
         __syncthreads();
         copy_out<R>(reg + REG_H1, H, S1, t);
@@ -330,11 +349,15 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_compress_h(const
             // the z >= 16 planes of H are free (the corner needs z < 16 only): they
             // stage this half's survivors when they fit, else these go out directly
             uint32_t i0 = corner ? 0u : cnt[256 + base], i1 = corner ? cnt[base] : cnt[512 + base];
-            R *st = S0 <= (uint32_t)STAGE0 ? H + STAGE0_AT : reg + REG_H0;
+            if (S0 <= (uint32_t)STAGE0) {
+                stage_row<R>(H + STAGE0_AT, H + STAGE0_AT + STAGE0 + (t & 63), bits, i0, i1, r, zkeep);
+            } else {
+                R *st = reg + REG_H0;
 #pragma unroll
-            for (int z = 0; z < 32; ++z) {
-                uint32_t &i = z < 16 ? i0 : i1;
-                if ((bits >> z) & 1u) st[i++] = zero_low_bits(r[z], zkeep);
+                for (int z = 0; z < 32; ++z) {
+                    uint32_t &i = z < 16 ? i0 : i1;
+                    if ((bits >> z) & 1u) st[i++] = zero_low_bits(r[z], zkeep);
+                }
             }
         }
         // levels 16 and 8 on the corner (the barrier inside orders the column stores above)
