@@ -171,10 +171,21 @@ static_assert(16384 + 64 <= HALF_WORDS, "room for the x >= 16 half's stage and i
 // i1 (z >= 16), zerobits applied.  Groups of eight positions that no row of
 // the wave keeps are skipped on a scalar test; a kept group runs branch-free,
 // unkept values going to the lane's dummy word (a test per position costs
-// taken branches).
-template <typename R>
+// taken branches).  Wavelet type 2 (the lifting step holds more registers)
+// keeps the per-position form: the grouped one spills there (binary64
+// compress +10%).
+template <typename R, int WT>
 CZW_DI void stage_row(R *st, R *dummy, uint32_t bits, uint32_t i0, uint32_t i1, const R *r, uint32_t zkeep)
 {
+    if constexpr (WT == 2) {
+        (void)dummy;
+#pragma unroll
+        for (int z = 0; z < 32; ++z) {
+            uint32_t &i = z < 16 ? i0 : i1;
+            if ((bits >> z) & 1u) st[i++] = zero_low_bits(r[z], zkeep);
+        }
+        return;
+    }
     const uint32_t wk = wave_or(bits);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -292,7 +303,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_compress_h(const
             }
             // every column is in registers: H becomes the stage of this half's
             // survivors (at most 16384), copied out with 16-byte stores
-            stage_row<R>(H, H + 16384 + (t & 63), bits, cnt[ri * 256 + base], cnt[(ri + 1) * 256 + base], r, zkeep);
+            stage_row<R, WT>(H, H + 16384 + (t & 63), bits, cnt[ri * 256 + base], cnt[(ri + 1) * 256 + base], r, zkeep);
         }
         __syncthreads();
         copy_out<R>(reg + REG_H1, H, S1, t);
@@ -350,7 +361,7 @@ __global__ __launch_bounds__(NT, sizeof(R) == 4 ? 4 : 2) void k_compress_h(const
             // stage this half's survivors when they fit, else these go out directly
             uint32_t i0 = corner ? 0u : cnt[256 + base], i1 = corner ? cnt[base] : cnt[512 + base];
             if (S0 <= (uint32_t)STAGE0) {
-                stage_row<R>(H + STAGE0_AT, H + STAGE0_AT + STAGE0 + (t & 63), bits, i0, i1, r, zkeep);
+                stage_row<R, WT>(H + STAGE0_AT, H + STAGE0_AT + STAGE0 + (t & 63), bits, i0, i1, r, zkeep);
             } else {
                 R *st = reg + REG_H0;
 #pragma unroll
