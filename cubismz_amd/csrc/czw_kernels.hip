@@ -238,6 +238,11 @@ constexpr int NRUN32 = 1792;  // level-32 stream runs: 7 codes x 16 x 16 runs of
 constexpr int STAGE32_AT = 16 * SP;  // the z >= 16 half of A
 constexpr int STAGE32 = ((LDS_WORDS - STAGE32_AT - 64) / 4) * 4;  // staged survivors (64 dummy words follow)
 constexpr int OVF_WORDS = 7 * 4096;  // level-32 survivors of a block at most
+// REC: a block's mask and staged survivors are parked in an LDS buffer of
+// this many words at the start of the next block, and written one block
+// later still, once the next block has found its record offset in the
+// background (the LDS the slot path leaves free)
+constexpr int SB_WORDS = 3904;
 constexpr int MAX_GROUPS = 256;      // compressor workgroups per launch at most (overflow areas)
 static_assert(STAGE32_AT >= CORNER_MAX, "corner survivors are staged below the level-32 ones");
 
@@ -254,6 +259,21 @@ struct CompressOut {
     int queue;          // slot mode: take blocks from the queue in hdr[0] instead of a static stride
 };
 
+// Block b's record offset e (ok) or its give-up: status word, offsets, error bit.
+CZW_DI void publish_offset(const CompressOut &o, int b, int nblocks, bool ok, uint64_t e, uint32_t n)
+{
+    if (ok) {
+        if (b > 0) st_store(o.status + b, ST_P | (e + 4u + n));
+        o.offsets[b] = e;
+        if (b == nblocks - 1) o.offsets[nblocks] = e + 4u + n;
+    } else {
+        st_store(o.status + b, ST_E);
+        atomicOr(o.hdr + 1, (uint32_t)CZW_ERR_LOOKBACK);
+        o.offsets[b] = CZW_OFFSET_INVALID;
+        if (b == nblocks - 1) o.offsets[nblocks] = CZW_OFFSET_INVALID;
+    }
+}
+
 template <int WT, int LAYOUT, bool REC, int NTH>
 __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ src, FieldGeom g, int nblocks,
                                                       float eps, CompressOut out,
@@ -268,6 +288,8 @@ __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ 
     __shared__ uint32_t s32_total;
     __shared__ int tk[3];  // REC: block tickets, one iteration ahead of their use
     __shared__ uint64_t e_sh;  // record offset of the pending block
+    __shared__ __align__(16) uint32_t SB[REC ? SB_WORDS : 1];  // REC: a parked block's mask and survivors
+    __shared__ uint64_t e_q;   // REC: the parked block's record offset
     __shared__ czw_field_t T[LAYOUT == LAYOUT_BATCH ? BATCH_MAX_FIELDS : 1];
     const int t = threadIdx.x;
     const int G = gridDim.x;
@@ -299,6 +321,8 @@ __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ 
     // on the last wave; slot mode: straight to its slot.
     int bp = -1;
     uint32_t pSc = 0, pS32 = 0;
+    int qb = -1;  // REC: the block parked in SB, framed at the start of the next iteration
+    uint32_t qSc = 0, qS32 = 0;
     for (int it = 0;; ++it) {
         const bool have = b < nblocks;
         if (!have && bp < 0) break;
@@ -306,72 +330,84 @@ __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ 
         // end of this block, so that no wave waits for the atomic's return
         int tkv = 0;
         if (queue && have && t == 0) tkv = (int)atomicAdd(out.hdr, 1u);
-        if (bp >= 0) {
-            const uint32_t S = pSc + pS32;
-            const uint32_t n = MASK_BYTES + 4u * S;
-            if constexpr (REC) {
-                if (t >= NTH - 64) {
-                    LookBack lb;
-                    look_back_issue(out.status, bp - 1, lb);
-                    uint64_t e = 0;
-                    const bool ok = look_back(out.status, bp, out.limit, lb, &e);
-                    if (t == NTH - 64) {
-                        if (ok) {
-                            if (bp > 0) st_store(out.status + bp, ST_P | (e + 4u + n));
-                            out.offsets[bp] = e;
-                            if (bp == nblocks - 1) out.offsets[nblocks] = e + 4u + n;
-                        } else {
-                            st_store(out.status + bp, ST_E);
-                            atomicOr(out.hdr + 1, (uint32_t)CZW_ERR_LOOKBACK);
-                            out.offsets[bp] = CZW_OFFSET_INVALID;
-                            if (bp == nblocks - 1) out.offsets[nblocks] = CZW_OFFSET_INVALID;
-                        }
-                        e_sh = ok ? e : CZW_OFFSET_INVALID;
-                    }
+        if constexpr (REC) {
+            const uint32_t zk = out.zkeep;
+            // block bk's record at offset e: size word, mask mk, survivors
+            // sv[0 .. SL) and the level-32 overflow area for the rest
+            auto frame = [&](uint64_t e, uint32_t S, uint32_t SL, uint32_t Sc, const uint32_t *mk,
+                             const uint32_t *sv) {
+                const uint32_t n = MASK_BYTES + 4u * S;
+                const uint32_t *ov = out.ovf + (size_t)blockIdx.x * OVF_WORDS;
+                uint32_t *rec = reinterpret_cast<uint32_t *>(out.dst + e);
+                if (t == 0) rec[0] = n;
+#pragma unroll
+                for (int k = 0; k < CR; ++k) rec[1 + t + NTH * k] = mk[t + NTH * k];
+                if (!out.shuffle) {
+                    for (uint32_t i = t; i < SL; i += NTH) rec[1 + 1024 + i] = sv[i] & zk;
+                    for (uint32_t i = SL + t; i < S; i += NTH) rec[1 + 1024 + i] = ov[i - Sc] & zk;
+                } else {  // byte p of survivor i at p*S + i (WaveletCompressor.cpp:56-73)
+                    uint8_t *p8 = out.dst + e + 4 + MASK_BYTES;
+                    auto put = [&](uint32_t i, uint32_t w) {
+                        p8[i] = (uint8_t)w;
+                        p8[S + i] = (uint8_t)(w >> 8);
+                        p8[2 * S + i] = (uint8_t)(w >> 16);
+                        p8[3 * S + i] = (uint8_t)(w >> 24);
+                    };
+                    for (uint32_t i = t; i < SL; i += NTH) put(i, sv[i] & zk);
+                    for (uint32_t i = SL + t; i < S; i += NTH) put(i, ov[i - Sc] & zk);
                 }
-                __syncthreads();
+            };
+            // block qb, parked in SB one block ago: its offset was found
+            // during that block (e_q), so no wave waits for it here
+            if (qb >= 0) {
+                const uint64_t e = e_q;
+                if (e != CZW_OFFSET_INVALID) frame(e, qSc + qS32, qSc + qS32, qSc, SB, SB + 1024);
+                qb = -1;
             }
+            __syncthreads();  // SB is read; bp's corner survivors were staged after its last barrier
+            if (bp >= 0) {
+                const uint32_t S = pSc + pS32;
+                const uint32_t *st = reinterpret_cast<const uint32_t *>(A) + (STAGE32_AT - pSc);
+                if (have && pS32 <= (uint32_t)STAGE32 && S <= (uint32_t)(SB_WORDS - 1024)) {
+                    // park bp in SB: its offset is found during the next block
+#pragma unroll
+                    for (int k = 0; k < CR; ++k) SB[t + NTH * k] = M[t + NTH * k];
+                    for (uint32_t i = t; i < S; i += NTH) SB[1024 + i] = st[i];
+                    qb = bp;
+                    qSc = pSc;
+                    qS32 = pS32;
+                } else {  // the last block, or too dense to park: look back now
+                    const uint32_t n = MASK_BYTES + 4u * S;
+                    if (t >= NTH - 64) {
+                        LookBack lb;
+                        look_back_issue(out.status, bp - 1, lb);
+                        uint64_t e = 0;
+                        const bool ok = look_back(out.status, bp, out.limit, lb, &e);
+                        if (t == NTH - 64) {
+                            publish_offset(out, bp, nblocks, ok, e, n);
+                            e_sh = ok ? e : CZW_OFFSET_INVALID;
+                        }
+                    }
+                    __syncthreads();
+                    const uint64_t e = e_sh;
+                    if (e != CZW_OFFSET_INVALID) frame(e, S, pS32 <= (uint32_t)STAGE32 ? S : pSc, pSc, M, st);
+                }
+                __syncthreads();  // A, M, SB and e_sh are free
+                bp = -1;
+            }
+        } else if (bp >= 0) {
             const bool staged = pS32 <= (uint32_t)STAGE32;
             const uint32_t *st = reinterpret_cast<const uint32_t *>(A) + (STAGE32_AT - pSc);
             const uint32_t zk = out.zkeep;
-            if constexpr (REC) {
-                const uint64_t e = e_sh;
-                if (e != CZW_OFFSET_INVALID) {
-                    // survivors staged in LDS (the corner's always; the level-32
-                    // ones unless the block was too dense, then in the overflow area)
-                    const uint32_t *ov = out.ovf + (size_t)blockIdx.x * OVF_WORDS;
-                    const uint32_t SL = staged ? S : pSc;
-                    uint32_t *rec = reinterpret_cast<uint32_t *>(out.dst + e);
-                    if (t == 0) rec[0] = n;
+            // the corner survivors were staged after the block's last barrier
+            __syncthreads();
+            uint32_t *slot = reinterpret_cast<uint32_t *>(out.dst + (size_t)bp * SLOT32);
 #pragma unroll
-                    for (int k = 0; k < CR; ++k) rec[1 + t + NTH * k] = M[t + NTH * k];
-                    if (!out.shuffle) {
-                        for (uint32_t i = t; i < SL; i += NTH) rec[1 + 1024 + i] = st[i] & zk;
-                        for (uint32_t i = SL + t; i < S; i += NTH) rec[1 + 1024 + i] = ov[i - pSc] & zk;
-                    } else {  // byte p of survivor i at p*S + i (WaveletCompressor.cpp:56-73)
-                        uint8_t *p8 = out.dst + e + 4 + MASK_BYTES;
-                        auto put = [&](uint32_t i, uint32_t w) {
-                            p8[i] = (uint8_t)w;
-                            p8[S + i] = (uint8_t)(w >> 8);
-                            p8[2 * S + i] = (uint8_t)(w >> 16);
-                            p8[3 * S + i] = (uint8_t)(w >> 24);
-                        };
-                        for (uint32_t i = t; i < SL; i += NTH) put(i, st[i] & zk);
-                        for (uint32_t i = SL + t; i < S; i += NTH) put(i, ov[i - pSc] & zk);
-                    }
-                }
-            } else {
-                // the corner survivors were staged after the block's last
-                // barrier (REC has the look-back's barrier above)
-                __syncthreads();
-                uint32_t *slot = reinterpret_cast<uint32_t *>(out.dst + (size_t)bp * SLOT32);
-#pragma unroll
-                for (int k = 0; k < CR; ++k) slot[t + NTH * k] = M[t + NTH * k];
-                for (uint32_t i = t; i < pSc; i += NTH) slot[1024 + i] = st[i] & zk;
-                if (staged)
-                    for (uint32_t i = t; i < pS32; i += NTH) slot[1024 + CORNER_MAX + i] = st[pSc + i] & zk;
-            }
-            __syncthreads();  // A, M and e_sh are free
+            for (int k = 0; k < CR; ++k) slot[t + NTH * k] = M[t + NTH * k];
+            for (uint32_t i = t; i < pSc; i += NTH) slot[1024 + i] = st[i] & zk;
+            if (staged)
+                for (uint32_t i = t; i < pS32; i += NTH) slot[1024 + CORNER_MAX + i] = st[pSc + i] & zk;
+            __syncthreads();  // A and M are free
             bp = -1;
         }
         if (!have) break;
@@ -535,9 +571,14 @@ __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ 
         __syncthreads();
         // Levels 16 and 8 on the [0,16)^3 corner keep 256, then 64 threads
         // busy; every thread issues its last prefetch quarter in a pass where
-        // it has no rows.
+        // it has no rows.  REC: the last wave meanwhile finds the parked
+        // block's record offset (its status reads issued here, evaluated
+        // after the level-16 passes).
         {
             float c[16];
+            LookBack lbq;
+            if constexpr (REC)
+                if (qb >= 0 && t >= NTH - 64) look_back_issue(out.status, qb - 1, lbq);
             if (t < 256) {  // level 16 x-pass: row (z, y) in [0,16)^2
                 const int z = t >> 4, yy = t & 15;
                 lds_ld<16>(A, z * SP + yy * RP, 1, c);
@@ -572,6 +613,16 @@ __global__ __launch_bounds__(NTH) void k_compress_f32(const float *__restrict__ 
                 }
             } else if (t < 256) {
                 prefetch(std::integral_constant<int, 24>());
+            }
+            if constexpr (REC) {
+                if (qb >= 0 && t >= NTH - 64) {
+                    uint64_t e = 0;
+                    const bool ok = look_back(out.status, qb, out.limit, lbq, &e);
+                    if (t == NTH - 64) {
+                        publish_offset(out, qb, nblocks, ok, e, MASK_BYTES + 4u * (qSc + qS32));
+                        e_q = ok ? e : CZW_OFFSET_INVALID;
+                    }
+                }
             }
             __syncthreads();
             if (t < 64) prefetch(std::integral_constant<int, 24>());
