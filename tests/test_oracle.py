@@ -39,7 +39,7 @@ def blocks(demo):
 
 
 def test_demo_fixture_matches_reference_data(demo):
-    # demo.h5 values lie in [0.00677, 0.8593] (survey of Tests/Data/data.tar.xz)
+    # demo.h5 values lie in [0.00677, 0.8593] (the range read from Tests/Data/data.tar.xz)
     assert demo["f32"].min() == pytest.approx(0.0067658233, abs=1e-9)
     assert demo["f32"].max() == pytest.approx(0.8592596, abs=1e-7)
     assert np.allclose(demo["f64"], demo["f32"], atol=1e-7)
