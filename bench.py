@@ -471,7 +471,7 @@ def adjoint_leg(a, torch, fields):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+        print(f"bench.py: the benchmark runs on one GPU only (asked for --gpus {a.gpus}); "
               "blocks are independent, so N GPUs would each compress their own subdomain.", file=sys.stderr)
         sys.exit(2)
 
