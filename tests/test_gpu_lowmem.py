@@ -116,3 +116,45 @@ def test_lookback_give_up(torch_gpu):
         czw.set_lookback_limit(prev)
     assert errs > 0
     _same(torch, czw.compress_field_lowmem(f, 1e-3, 1, 0, False, scratch=scratch), want)
+
+
+PARK = 2880  # survivors of a block that can be parked in LDS between its compute and its record
+
+
+def _survivors(payload):
+    return int(np.unpackbits(np.frombuffer(payload[:4096], dtype=np.uint8)).sum())
+
+
+@pytest.mark.parametrize("target", [PARK - 1, PARK, PARK + 1])
+def test_parked_block_capacity(torch_gpu, oracle, target):
+    """Blocks with exactly PARK - 1, PARK and PARK + 1 survivors (eps chosen
+    from the oracle's mask), 640 copies of one block in a field so that every
+    workgroup compresses several and parks all but its last (or, above the
+    capacity, looks each up at once): records equal the two-stage path's,
+    and the first one the oracle's payload."""
+    torch = torch_gpu
+    rng = np.random.default_rng(target)
+    blk = rng.standard_normal((32, 32, 32)).astype(np.float32)
+    lo, hi = -8.0, 2.0  # log10 eps: many survivors at lo, few at hi
+    for _ in range(100):
+        mid = 0.5 * (lo + hi)
+        eps = float(np.float32(10.0 ** mid))
+        n = _survivors(oracle.compress(blk, eps, 1))
+        if n == target:
+            break
+        if n > target:
+            lo = mid
+        else:
+            hi = mid
+    else:
+        raise AssertionError(f"no eps gives {target} survivors")
+    print(f"[case] 640 copies of a block with {target} survivors, eps {eps:.6g}", flush=True)
+    one = np.ascontiguousarray(blk.transpose(2, 1, 0))  # f[x][y][z] = blk[z][y][x]
+    f = torch.from_numpy(np.tile(one, (8, 8, 10))).cuda()
+    a = czw.compress_field_lowmem(f, eps, 1)
+    b = czw.compress_field(f, eps, 1)
+    torch.cuda.synchronize()
+    _same(torch, a, b)
+    rec, off = a
+    first = czw.records_to_payloads(rec.cpu().numpy(), off.cpu().numpy())[0]
+    assert bytes(first) == oracle.compress(blk, eps, 1).tobytes()
